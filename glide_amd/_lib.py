@@ -1,0 +1,117 @@
+"""ctypes binding of ``libglide_hip.so`` (the C ABI declared in include/glide_hip.h).
+
+The library is required: importing anything that launches work raises if it
+has not been built (``python -m glide_amd.build``) - there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_void_p
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libglide_hip.so")
+
+GLIDE_E_NULL, GLIDE_E_SIZE, GLIDE_E_ARG, GLIDE_E_NONFINITE = -1, -2, -3, -4
+
+
+class GlideError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+class GlideParams(ctypes.Structure):
+    _fields_ = [("n", c_double), ("eps_reg", c_double), ("water_drag", c_double),
+                ("calving_rate", c_double), ("gl_sigmoid_c", c_double), ("gl_derivatives", c_int)]
+
+
+class GlideGeom(ctypes.Structure):
+    _fields_ = [("ny", c_int), ("nx", c_int), ("dx", c_double), ("dt", c_double)]
+
+
+DP = c_void_p   # device pointers travel as plain addresses
+
+_LEVEL_PTRS = ["U", "f", "r", "w", "z", "Lambda", "Lambda_out", "f_adj", "r_adj",
+               "H_prev", "bed", "B", "beta", "smb", "gamma", "chi", "mask", "scratch", "active"]
+
+
+class GlideLevel(ctypes.Structure):
+    _fields_ = [("g", GlideGeom), ("p", GlideParams)] + [(n, c_void_p) for n in _LEVEL_PTRS]
+
+
+class GlideSchedule(ctypes.Structure):
+    _fields_ = [("pre", c_int), ("post", c_int), ("coarse", c_int), ("local_passes", c_int),
+                ("local_sweeps", c_int), ("n_newton", c_int), ("adj_sweeps", c_int), ("adj_coarse", c_int)]
+
+
+# name -> (restype, argtypes); the list mirrors include/glide_hip.h
+SIGNATURES = {
+    "glide_residual": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP, DP, DP, c_void_p]),
+    "glide_jvp": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP, DP, c_void_p]),
+    "glide_vjp": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP, DP, c_void_p]),
+    "glide_vanka_sweep": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP,
+                                  c_int, c_int, DP, c_void_p]),
+    "glide_vanka_block": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, c_int, DP,
+                                  c_void_p]),
+    "glide_local_activation": (c_int, [POINTER(GlideGeom), DP, DP, DP, DP, c_void_p]),
+    "glide_vanka_adjoint_pass": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP,
+                                         DP, DP, c_int, c_double, c_void_p]),
+    "glide_grad_beta": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, c_int, c_void_p]),
+    "glide_restrict_u": (c_int, [DP, DP, c_int, c_int, c_void_p]),
+    "glide_restrict_v": (c_int, [DP, DP, c_int, c_int, c_void_p]),
+    "glide_restrict_cell": (c_int, [DP, DP, c_int, c_int, c_void_p]),
+    "glide_restrict_max": (c_int, [DP, DP, c_int, c_int, c_void_p]),
+    "glide_prolong_inject": (c_int, [DP, DP, c_int, c_int, c_double, c_void_p]),
+    "glide_prolong_u": (c_int, [DP, DP, c_int, c_int, c_void_p]),
+    "glide_prolong_v": (c_int, [DP, DP, c_int, c_int, c_void_p]),
+    "glide_prolong_H": (c_int, [DP, DP, c_int, c_int, c_void_p]),
+    "glide_restrict_parameters": (c_int, [POINTER(GlideLevel), c_int, c_void_p]),
+    "glide_propagate_geometry": (c_int, [POINTER(GlideLevel), c_int, c_double, c_void_p]),
+    "glide_fascd_vcycle": (c_int, [POINTER(GlideLevel), c_int, c_double, POINTER(GlideSchedule), c_void_p]),
+    "glide_forward_setup": (c_int, [POINTER(GlideLevel), c_int, c_double, c_void_p]),
+    "glide_residual_norm": (c_int, [POINTER(GlideLevel), POINTER(c_double), c_void_p]),
+    "glide_adjoint_vcycle": (c_int, [POINTER(GlideLevel), c_int, POINTER(GlideSchedule), c_void_p]),
+    "glide_last_error": (c_char_p, []),
+    "glide_version": (c_char_p, []),
+}
+
+_lib = None
+
+
+def load():
+    """Load the library (once).  Raises if it is missing: no fallback path."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing; build it with `python -m glide_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name in ("glide_last_error", "glide_version"):
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = SIGNATURES[name]
+    _lib = lib
+    return lib
+
+
+_bound: dict = {}
+
+
+def fn(name: str):
+    """The typed ctypes function for entry point ``name``."""
+    f = _bound.get(name)
+    if f is None:
+        f = getattr(load(), name)
+        f.restype, f.argtypes = SIGNATURES[name]
+        _bound[name] = f
+    return f
+
+
+def call(name: str, *args) -> None:
+    """Call a status-returning entry point and raise GlideError on failure."""
+    lib = load()
+    rc = fn(name)(*args)
+    if rc != 0:
+        raise GlideError(name, rc, lib.glide_last_error().decode(errors="replace"))

@@ -1,0 +1,949 @@
+// glide_hip.hip - MI355X (gfx950) kernels and C ABI of the SSA / FASCD solver.
+//
+// Layout: FP64 row-major MAC fields, state vector [u | v | H] per level.
+// Stencil kernels work on 64 x 4 point tiles (one wave64 per tile row, so
+// every row access is a contiguous 512-byte segment); the cell viscosity of
+// the tile plus a one-cell ring is computed once per tile into LDS.  The
+// Vanka smoother is one lane per cell (the 10-step local Newton stays in
+// registers) writing its 5 local updates to a scratch array; a second
+// gather pass applies them (each face sums its <= 2 adjacent cells' updates,
+// so no atomics are needed and the result is deterministic).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "glide_hip.h"
+#include "vanka_device.h"
+
+using namespace glide;
+
+namespace {
+
+constexpr int TX = 64;
+constexpr int TY = 4;
+constexpr int NT = TX * TY;
+constexpr int RB = 14;  // reference launch-block interior (glide/grid.py:184-186)
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail((int)e, std::string(what) + ": " + hipGetErrorString(e));
+    return GLIDE_OK;
+}
+
+__host__ __device__ inline size_t n_u(int ny, int nx) { return (size_t)ny * (nx + 1); }
+__host__ __device__ inline size_t n_v(int ny, int nx) { return (size_t)(ny + 1) * nx; }
+__host__ __device__ inline size_t n_c(int ny, int nx) { return (size_t)ny * nx; }
+__host__ __device__ inline size_t n_t(int ny, int nx) { return n_u(ny, nx) + n_v(ny, nx) + n_c(ny, nx); }
+
+int check_geom(const GlideGeom *g)
+{
+    if (!g) return fail(GLIDE_E_NULL, "geometry is NULL");
+    if (g->ny < 1 || g->nx < 1 || g->ny > (1 << 16) || g->nx > (1 << 16))
+        return fail(GLIDE_E_SIZE, "grid size out of range");
+    if (!(g->dx > 0.0) || !(g->dt > 0.0) || !std::isfinite(g->dx) || !std::isfinite(g->dt))
+        return fail(GLIDE_E_ARG, "dx and dt must be positive and finite");
+    return GLIDE_OK;
+}
+
+inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+dim3 point_grid(int ny, int nx) { return dim3((nx + 1 + TX - 1) / TX, (ny + 1 + TY - 1) / TY); }
+dim3 cell_grid(int ny, int nx) { return dim3((nx + TX - 1) / TX, (ny + TY - 1) / TY); }
+inline unsigned blocks1d(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// viscosity tile: cells [i0-1, i0+TY] x [j0-1, j0+TX] into LDS (0 outside)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void fill_eta_tile(double (*eta_s)[TX + 2], int i0, int j0, const double *__restrict__ u,
+                                              const double *__restrict__ v, const double *__restrict__ B,
+                                              const Params &p, double dx, int ny, int nx)
+{
+    int tid = threadIdx.y * TX + threadIdx.x;
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        int a = k / (TX + 2), b = k - a * (TX + 2);
+        int ci = i0 - 1 + a, cj = j0 - 1 + b;
+        eta_s[a][b] = (ci >= 0 && ci < ny && cj >= 0 && cj < nx) ? eta_cell(ci, cj, u, v, B, p, dx, ny, nx) : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Residual r = F(U) - f (- sub)  (glide/cuda/residuals.cu:6-292)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_residual(GlideGeom g, Params p, double *__restrict__ r,
+                                                 const double *__restrict__ U, const double *__restrict__ f,
+                                                 const double *__restrict__ sub, const double *__restrict__ bed,
+                                                 const double *__restrict__ B, const double *__restrict__ beta,
+                                                 const double *__restrict__ mask, const double *__restrict__ gamma)
+{
+    __shared__ double eta_s[TY + 2][TX + 2];
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv;
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    fill_eta_tile(eta_s, i0, j0, u, v, B, p, g.dx, ny, nx);
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    if (i > ny || j > nx) return;
+    auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+
+    if (i < ny && j < nx) {
+        double H_c = ld_c(H, i, j, ny, nx);
+        double rH = H_c / dt - (f ? ld_c(f + nu + nv, i, j, ny, nx) : 0.0);
+        Calv jc = calving(H_c, ld_c(bed, i, j, ny, nx), p);
+        rH -= jc.res;
+        double H_l = ld_c(H, i, j - 1, ny, nx);
+        Flux q = flux_x(ld_u(u, i, j, ny, nx), H_l, H_c, j, nx);
+        rH -= q.res * dx_inv;
+        double H_r = ld_c(H, i, j + 1, ny, nx);
+        q = flux_x(ld_u(u, i, j + 1, ny, nx), H_c, H_r, j + 1, nx);
+        rH += q.res * dx_inv;
+        double H_t = ld_c(H, i - 1, j, ny, nx);
+        q = flux_y(ld_v(v, i, j, ny, nx), H_t, H_c, i, ny);
+        rH += q.res * dx_inv;
+        double H_b = ld_c(H, i + 1, j, ny, nx);
+        q = flux_y(ld_v(v, i + 1, j, ny, nx), H_c, H_b, i + 1, ny);
+        rH -= q.res * dx_inv;
+        double m = ld_c(mask, i, j, ny, nx);
+        double th = ld_c(gamma, i, j, ny, nx);
+        double val = (1.0 - m) * rH + m * (H_c - th);
+        size_t k = nu + nv + (size_t)i * nx + j;
+        r[k] = sub ? val - sub[k] : val;
+    }
+    if (i < ny) {
+        double ru = f ? -ld_u(f, i, j, ny, nx) : -0.0;
+        {
+            double eH = E(i, j) * ld_c(H, i, j, ny, nx);
+            SigN s = sigma_xx(ld_u(u, i, j, ny, nx), ld_u(u, i, j + 1, ny, nx), ld_v(v, i, j, ny, nx),
+                              ld_v(v, i + 1, j, ny, nx), eH, dx_inv, j, nx);
+            ru += s.res * dx_inv;
+        }
+        {
+            double eH = E(i, j - 1) * ld_c(H, i, j - 1, ny, nx);
+            SigN s = sigma_xx(ld_u(u, i, j - 1, ny, nx), ld_u(u, i, j, ny, nx), ld_v(v, i, j - 1, ny, nx),
+                              ld_v(v, i + 1, j - 1, ny, nx), eH, dx_inv, j - 1, nx);
+            ru -= s.res * dx_inv;
+        }
+        {
+            double eH = etaH_vertex(E(i - 1, j - 1), E(i - 1, j), E(i, j - 1), E(i, j), ld_c(H, i - 1, j - 1, ny, nx),
+                                    ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx));
+            SigS s = sigma_xy(ld_u(u, i - 1, j, ny, nx), ld_u(u, i, j, ny, nx), ld_v(v, i, j - 1, ny, nx),
+                              ld_v(v, i, j, ny, nx), eH, dx_inv, i, j, ny, nx);
+            ru += s.res * dx_inv;
+        }
+        {
+            double eH = etaH_vertex(E(i, j - 1), E(i, j), E(i + 1, j - 1), E(i + 1, j), ld_c(H, i, j - 1, ny, nx),
+                                    ld_c(H, i, j, ny, nx), ld_c(H, i + 1, j - 1, ny, nx), ld_c(H, i + 1, j, ny, nx));
+            SigS s = sigma_xy(ld_u(u, i, j, ny, nx), ld_u(u, i + 1, j, ny, nx), ld_v(v, i + 1, j - 1, ny, nx),
+                              ld_v(v, i + 1, j, ny, nx), eH, dx_inv, i + 1, j, ny, nx);
+            ru -= s.res * dx_inv;
+        }
+        {
+            TauB t = tau_b(ld_u(u, i, j, ny, nx), ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx),
+                           ld_c(bed, i, j - 1, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i, j - 1, ny, nx),
+                           ld_c(beta, i, j, ny, nx), p);
+            ru += t.res;
+        }
+        {
+            TauD t = tau_dx(ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx), ld_c(bed, i, j - 1, ny, nx),
+                            ld_c(bed, i, j, ny, nx), dx_inv, j, nx);
+            ru -= t.res;
+        }
+        size_t k = (size_t)i * (nx + 1) + j;
+        r[k] = sub ? ru - sub[k] : ru;
+    }
+    if (j < nx) {
+        double rv = f ? -ld_v(f + nu, i, j, ny, nx) : -0.0;
+        {
+            double eH = E(i - 1, j) * ld_c(H, i - 1, j, ny, nx);
+            SigN s = sigma_yy(ld_u(u, i - 1, j, ny, nx), ld_u(u, i - 1, j + 1, ny, nx), ld_v(v, i - 1, j, ny, nx),
+                              ld_v(v, i, j, ny, nx), eH, dx_inv, i - 1, ny);
+            rv += s.res * dx_inv;
+        }
+        {
+            double eH = E(i, j) * ld_c(H, i, j, ny, nx);
+            SigN s = sigma_yy(ld_u(u, i, j, ny, nx), ld_u(u, i, j + 1, ny, nx), ld_v(v, i, j, ny, nx),
+                              ld_v(v, i + 1, j, ny, nx), eH, dx_inv, i, ny);
+            rv -= s.res * dx_inv;
+        }
+        {
+            double eH = etaH_vertex(E(i - 1, j - 1), E(i - 1, j), E(i, j - 1), E(i, j), ld_c(H, i - 1, j - 1, ny, nx),
+                                    ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx));
+            SigS s = sigma_xy(ld_u(u, i - 1, j, ny, nx), ld_u(u, i, j, ny, nx), ld_v(v, i, j - 1, ny, nx),
+                              ld_v(v, i, j, ny, nx), eH, dx_inv, i, j, ny, nx);
+            rv -= s.res * dx_inv;
+        }
+        {
+            double eH = etaH_vertex(E(i - 1, j), E(i - 1, j + 1), E(i, j), E(i, j + 1), ld_c(H, i - 1, j, ny, nx),
+                                    ld_c(H, i - 1, j + 1, ny, nx), ld_c(H, i, j, ny, nx), ld_c(H, i, j + 1, ny, nx));
+            SigS s = sigma_xy(ld_u(u, i - 1, j + 1, ny, nx), ld_u(u, i, j + 1, ny, nx), ld_v(v, i, j, ny, nx),
+                              ld_v(v, i, j + 1, ny, nx), eH, dx_inv, i, j + 1, ny, nx);
+            rv += s.res * dx_inv;
+        }
+        {
+            TauB t = tau_b(ld_v(v, i, j, ny, nx), ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j, ny, nx),
+                           ld_c(bed, i - 1, j, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i - 1, j, ny, nx),
+                           ld_c(beta, i, j, ny, nx), p);
+            rv += t.res;
+        }
+        {
+            TauD t = tau_dy(ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j, ny, nx), ld_c(bed, i - 1, j, ny, nx),
+                            ld_c(bed, i, j, ny, nx), dx_inv, i, ny);
+            rv -= t.res;
+        }
+        size_t k = nu + (size_t)i * nx + j;
+        r[k] = sub ? rv - sub[k] : rv;
+    }
+}
+
+template <class EtaFn>
+__device__ __forceinline__ CellNbhd load_nbhd(int i, int j, int ny, int nx, size_t nu, size_t nv,
+                                              const double *__restrict__ u, const double *__restrict__ v,
+                                              const double *__restrict__ H, const double *__restrict__ f,
+                                              const double *__restrict__ bed, const double *__restrict__ beta,
+                                              const EtaFn &E)
+{
+    CellNbhd N;
+    N.e_c = E(i, j); N.e_l = E(i, j - 1); N.e_r = E(i, j + 1); N.e_t = E(i - 1, j); N.e_b = E(i + 1, j);
+    N.e_tl = E(i - 1, j - 1); N.e_tr = E(i - 1, j + 1); N.e_bl = E(i + 1, j - 1); N.e_br = E(i + 1, j + 1);
+    N.H_l = ld_c(H, i, j - 1, ny, nx); N.H_r = ld_c(H, i, j + 1, ny, nx);
+    N.H_t = ld_c(H, i - 1, j, ny, nx); N.H_b = ld_c(H, i + 1, j, ny, nx);
+    N.H_tl = ld_c(H, i - 1, j - 1, ny, nx); N.H_tr = ld_c(H, i - 1, j + 1, ny, nx);
+    N.H_bl = ld_c(H, i + 1, j - 1, ny, nx); N.H_br = ld_c(H, i + 1, j + 1, ny, nx);
+    N.bed_c = ld_c(bed, i, j, ny, nx); N.bed_l = ld_c(bed, i, j - 1, ny, nx); N.bed_r = ld_c(bed, i, j + 1, ny, nx);
+    N.bed_t = ld_c(bed, i - 1, j, ny, nx); N.bed_b = ld_c(bed, i + 1, j, ny, nx);
+    N.beta_c = ld_c(beta, i, j, ny, nx); N.beta_l = ld_c(beta, i, j - 1, ny, nx);
+    N.beta_r = ld_c(beta, i, j + 1, ny, nx); N.beta_t = ld_c(beta, i - 1, j, ny, nx);
+    N.beta_b = ld_c(beta, i + 1, j, ny, nx);
+    N.u_ll = ld_u(u, i, j - 1, ny, nx); N.u_rr = ld_u(u, i, j + 2, ny, nx);
+    N.u_tl = ld_u(u, i - 1, j, ny, nx); N.u_tr = ld_u(u, i - 1, j + 1, ny, nx);
+    N.u_bl = ld_u(u, i + 1, j, ny, nx); N.u_br = ld_u(u, i + 1, j + 1, ny, nx);
+    N.v_lt = ld_v(v, i, j - 1, ny, nx); N.v_lb = ld_v(v, i + 1, j - 1, ny, nx);
+    N.v_rt = ld_v(v, i, j + 1, ny, nx); N.v_rb = ld_v(v, i + 1, j + 1, ny, nx);
+    N.v_tt = ld_v(v, i - 1, j, ny, nx); N.v_bb = ld_v(v, i + 2, j, ny, nx);
+    if (f) {
+        N.f0 = ld_u(f, i, j, ny, nx); N.f1 = ld_u(f, i, j + 1, ny, nx);
+    N.f2 = ld_v(f + nu, i, j, ny, nx); N.f3 = ld_v(f + nu, i + 1, j, ny, nx);
+    N.f4 = ld_v(f + nu + nv, i, j, ny, nx);
+    } else {
+        N.f0 = N.f1 = N.f2 = N.f3 = N.f4 = 0.0;
+    }
+
+    return N;
+}
+
+// ---------------------------------------------------------------------------
+// Forward Vanka: per-cell damped Newton (glide/cuda/vanka.cu:3-439, 844-1295)
+// cd (5 x nc, structure of arrays) receives 0.5*(x - x0) for the four faces
+// and (H - H0) for the cell.
+// ---------------------------------------------------------------------------
+template <bool LOCAL>
+__global__ __launch_bounds__(NT) void k_vanka(GlideGeom g, Params p, const double *__restrict__ U,
+                                              const double *__restrict__ f, const double *__restrict__ bed,
+                                              const double *__restrict__ B, const double *__restrict__ beta,
+                                              const double *__restrict__ gamma, const unsigned char *__restrict__ blk,
+                                              int nbx, int n_newton, double *__restrict__ cd)
+{
+    __shared__ double eta_s[TY + 2][TX + 2];
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv;
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    bool mine = (i < ny && j < nx);
+    if (LOCAL) {
+        mine = mine && blk[(i / RB) * nbx + j / RB];
+        if (!__syncthreads_or(mine)) return;   // uniform: the whole tile is idle
+    }
+    fill_eta_tile(eta_s, i0, j0, u, v, B, p, g.dx, ny, nx);
+    __syncthreads();
+    if (!mine) return;
+    auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+
+    CellNbhd N = load_nbhd(i, j, ny, nx, nu, nv, u, v, H, f, bed, beta, E);
+    const double u_l0 = ld_u(u, i, j, ny, nx), u_r0 = ld_u(u, i, j + 1, ny, nx);
+    const double v_t0 = ld_v(v, i, j, ny, nx), v_b0 = ld_v(v, i + 1, j, ny, nx);
+    const double H_c0 = ld_c(H, i, j, ny, nx), th = ld_c(gamma, i, j, ny, nx);
+    double u_l = u_l0, u_r = u_r0, v_t = v_t0, v_b = v_b0, H_c = H_c0;
+    for (int k = 0; k < n_newton; k++) {
+        double J[25], r[5], d[5];
+        vanka_assemble<0>(N, p, i, j, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, false, J, r);
+        lu5_solve(J, r, d);
+        u_l -= 0.25 * d[0];
+        u_r -= 0.25 * d[1];
+        v_t -= 0.25 * d[2];
+        v_b -= 0.25 * d[3];
+        H_c -= 0.25 * d[4];
+        H_c = fmax(H_c, th);
+    }
+    const size_t ci = (size_t)i * nx + j;
+    cd[ci] = 0.5 * (u_l - u_l0);
+    cd[nc + ci] = 0.5 * (u_r - u_r0);
+    cd[2 * nc + ci] = 0.5 * (v_t - v_t0);
+    cd[3 * nc + ci] = 0.5 * (v_b - v_b0);
+    cd[4 * nc + ci] = H_c - H_c0;
+}
+
+
+// Local block (J, r) of every cell after k Newton updates (test hook):
+// out[30*ci + 0..24] = J, out[30*ci + 25..29] = r.
+__global__ __launch_bounds__(NT) void k_vanka_block(GlideGeom g, Params p, const double *__restrict__ U,
+                                                    const double *__restrict__ f, const double *__restrict__ bed,
+                                                    const double *__restrict__ B, const double *__restrict__ beta,
+                                                    const double *__restrict__ gamma, int k_updates,
+                                                    double *__restrict__ out)
+{
+    __shared__ double eta_s[TY + 2][TX + 2];
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv;
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    fill_eta_tile(eta_s, i0, j0, u, v, B, p, g.dx, ny, nx);
+    __syncthreads();
+    if (i >= ny || j >= nx) return;
+    auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+    CellNbhd N = load_nbhd(i, j, ny, nx, nu, nv, u, v, H, f, bed, beta, E);
+    double u_l = ld_u(u, i, j, ny, nx), u_r = ld_u(u, i, j + 1, ny, nx);
+    double v_t = ld_v(v, i, j, ny, nx), v_b = ld_v(v, i + 1, j, ny, nx);
+    double H_c = ld_c(H, i, j, ny, nx);
+    const double th = ld_c(gamma, i, j, ny, nx);
+    double J[25], r[5], d[5];
+    for (int k = 0; k <= k_updates; k++) {
+        vanka_assemble<0>(N, p, i, j, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, false, J, r);
+        if (k == k_updates) break;
+        lu5_solve(J, r, d);
+        u_l -= 0.25 * d[0];
+        u_r -= 0.25 * d[1];
+        v_t -= 0.25 * d[2];
+        v_b -= 0.25 * d[3];
+        H_c -= 0.25 * d[4];
+        H_c = fmax(H_c, th);
+    }
+    double *o = out + 30 * ((size_t)i * nx + j);
+    for (int q = 0; q < 25; q++) o[q] = J[q];
+    for (int q = 0; q < 5; q++) o[25 + q] = r[q];
+}
+
+// Gather the per-cell updates: U += 0.5 * delta (glide/grid.py:305-317)
+template <bool LOCAL>
+__global__ __launch_bounds__(NT) void k_vanka_apply(int ny, int nx, double *__restrict__ U,
+                                                    const double *__restrict__ cd,
+                                                    const unsigned char *__restrict__ blk, int nbx)
+{
+    const int i = blockIdx.y * TY + threadIdx.y, j = blockIdx.x * TX + threadIdx.x;
+    if (i > ny || j > nx) return;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
+    auto act = [&](int ci, int cj) { return !LOCAL || blk[(ci / RB) * nbx + cj / RB] != 0; };
+    if (i < ny) {
+        double d = 0.0;
+        if (j > 0 && act(i, j - 1)) d = d + cd[nc + (size_t)i * nx + j - 1];
+        if (j < nx && act(i, j)) d = d + cd[(size_t)i * nx + j];
+        U[(size_t)i * (nx + 1) + j] += 0.5 * d;
+    }
+    if (j < nx) {
+        double d = 0.0;
+        if (i > 0 && act(i - 1, j)) d = d + cd[3 * nc + (size_t)(i - 1) * nx + j];
+        if (i < ny && act(i, j)) d = d + cd[2 * nc + (size_t)i * nx + j];
+        U[nu + (size_t)i * nx + j] += 0.5 * d;
+    }
+    if (i < ny && j < nx) {
+        size_t ci = (size_t)i * nx + j;
+        double d = act(i, j) ? cd[4 * nc + ci] : 0.0;
+        U[nu + nv + ci] += 0.5 * d;
+    }
+}
+
+// Local-smoother activation: one workgroup per 14x14 reference block reads
+// its 16x16 footprint (glide/solver.py:148-154, glide/cuda/vanka.cu:870-888).
+__global__ __launch_bounds__(256) void k_local_activation(int ny, int nx, const double *__restrict__ r,
+                                                          const double *__restrict__ U,
+                                                          const double *__restrict__ gamma,
+                                                          unsigned char *__restrict__ blk)
+{
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const int i = by * RB - 1 + (int)threadIdx.y, j = bx * RB - 1 + (int)threadIdx.x;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    int bad = 0;
+    if (i >= 0 && i < ny && j >= 0 && j < nx) {
+        size_t ci = (size_t)i * nx + j;
+        double a = r[nu + nv + ci], b = U[nu + nv + ci] - gamma[ci];
+        double fb = a + b - sqrt(a * a + b * b);
+        double te = fabs(r[(size_t)i * (nx + 1) + j + 1]) + fabs(r[(size_t)i * (nx + 1) + j]) +
+                    fabs(r[nu + (size_t)(i + 1) * nx + j]) + fabs(r[nu + (size_t)i * nx + j]) + fabs(fb);
+        bad = te > 0.01;
+    }
+    int any = __syncthreads_or(bad);
+    if (threadIdx.x == 0 && threadIdx.y == 0) blk[by * gridDim.x + bx] = (unsigned char)(any != 0);
+}
+
+// ---------------------------------------------------------------------------
+// Transfer operators (glide/cuda/utility.cu)
+// ---------------------------------------------------------------------------
+__global__ void k_restrict_u(const double *__restrict__ uf, double *__restrict__ uc, double *__restrict__ uc2,
+                             int nyc, int nxc)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nyc * (nxc + 1)) return;
+    int J = idx / (nxc + 1), I = idx - J * (nxc + 1);
+    double val = 0.5 * (uf[(size_t)(2 * J) * (2 * nxc + 1) + 2 * I] + uf[(size_t)(2 * J + 1) * (2 * nxc + 1) + 2 * I]);
+    uc[idx] = val;
+    if (uc2) uc2[idx] = val;
+}
+__global__ void k_restrict_v(const double *__restrict__ vf, double *__restrict__ vc, double *__restrict__ vc2,
+                             int nyc, int nxc)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (nyc + 1) * nxc) return;
+    int J = idx / nxc, I = idx - J * nxc;
+    double val = 0.5 * (vf[(size_t)(2 * J) * (2 * nxc) + 2 * I] + vf[(size_t)(2 * J) * (2 * nxc) + 2 * I + 1]);
+    vc[idx] = val;
+    if (vc2) vc2[idx] = val;
+}
+__global__ void k_restrict_cell(const double *__restrict__ hf, double *__restrict__ hc, double *__restrict__ hc2,
+                                int nyc, int nxc)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nyc * nxc) return;
+    int J = idx / nxc, I = idx - J * nxc;
+    size_t nxf = 2 * (size_t)nxc;
+    double val = 0.25 * (hf[(2 * J) * nxf + 2 * I] + hf[(2 * J) * nxf + 2 * I + 1] + hf[(2 * J + 1) * nxf + 2 * I] +
+                         hf[(2 * J + 1) * nxf + 2 * I + 1]);
+    hc[idx] = val;
+    if (hc2) hc2[idx] = val;
+}
+__global__ void k_restrict_max(const double *__restrict__ hf, double *__restrict__ hc, int nyc, int nxc)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nyc * nxc) return;
+    int J = idx / nxc, I = idx - J * nxc;
+    size_t nxf = 2 * (size_t)nxc;
+    double a = fmax(hf[(2 * J) * nxf + 2 * I], hf[(2 * J) * nxf + 2 * I + 1]);
+    double b = fmax(hf[(2 * J + 1) * nxf + 2 * I], hf[(2 * J + 1) * nxf + 2 * I + 1]);
+    hc[idx] = fmax(a, b);
+}
+__global__ void k_prolong_inject(const double *__restrict__ hc, double *__restrict__ hf, int nyf, int nxf, double sign)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nyf * nxf) return;
+    int j = idx / nxf, i = idx - j * nxf;
+    hf[idx] = sign * hc[(size_t)(j / 2) * (nxf / 2) + i / 2];
+}
+
+__device__ __forceinline__ double bilin(double ty, double tx, double v00, double v01, double v10, double v11)
+{
+    return (1.0 - ty) * ((1.0 - tx) * v00 + tx * v01) + ty * ((1.0 - tx) * v10 + tx * v11);
+}
+// coarse sample helper: value of (a[k] - b[k]) when b != NULL, else a[k]
+__device__ __forceinline__ double cdiff(const double *__restrict__ a, const double *__restrict__ b, size_t k)
+{
+    return b ? a[k] - b[k] : a[k];
+}
+// bilinear u prolongation of (uc - wc) at fine face (j, i) (utility.cu:180-232)
+__device__ __forceinline__ double prol_u_at(const double *uc, const double *wc, int j, int i, int nyc, int nxc)
+{
+    double Jf = (j - 0.5) * 0.5, If = i * 0.5;
+    Jf = fmax(0.0, fmin(Jf, (double)(nyc - 1)));
+    If = fmax(0.0, fmin(If, (double)nxc));
+    int Jl = (int)Jf, Il = (int)If;
+    int Jh = min(Jl + 1, nyc - 1), Ih = min(Il + 1, nxc);
+    double ty = Jf - Jl, tx = If - Il;
+    size_t s = nxc + 1;
+    return bilin(ty, tx, cdiff(uc, wc, Jl * s + Il), cdiff(uc, wc, Jl * s + Ih), cdiff(uc, wc, Jh * s + Il),
+                 cdiff(uc, wc, Jh * s + Ih));
+}
+// utility.cu:235-287
+__device__ __forceinline__ double prol_v_at(const double *vc, const double *wc, int j, int i, int nyc, int nxc)
+{
+    double Jf = j * 0.5, If = (i - 0.5) * 0.5;
+    Jf = fmax(0.0, fmin(Jf, (double)nyc));
+    If = fmax(0.0, fmin(If, (double)(nxc - 1)));
+    int Jl = (int)Jf, Il = (int)If;
+    int Jh = min(Jl + 1, nyc), Ih = min(Il + 1, nxc - 1);
+    double ty = Jf - Jl, tx = If - Il;
+    size_t s = nxc;
+    return bilin(ty, tx, cdiff(vc, wc, Jl * s + Il), cdiff(vc, wc, Jl * s + Ih), cdiff(vc, wc, Jh * s + Il),
+                 cdiff(vc, wc, Jh * s + Ih));
+}
+// utility.cu:289-337
+__device__ __forceinline__ double prol_H_at(const double *hc, const double *wc, int j, int i, int nyc, int nxc)
+{
+    double Jf = (j - 0.5) * 0.5, If = (i - 0.5) * 0.5;
+    Jf = fmax(0.0, fmin(Jf, (double)(nyc - 1)));
+    If = fmax(0.0, fmin(If, (double)(nxc - 1)));
+    int Jl = (int)Jf, Il = (int)If;
+    int Jh = min(Jl + 1, nyc - 1), Ih = min(Il + 1, nxc - 1);
+    double ty = Jf - Jl, tx = If - Il;
+    size_t s = nxc;
+    return bilin(ty, tx, cdiff(hc, wc, Jl * s + Il), cdiff(hc, wc, Jl * s + Ih), cdiff(hc, wc, Jh * s + Il),
+                 cdiff(hc, wc, Jh * s + Ih));
+}
+__global__ void k_prolong_u(const double *__restrict__ uc, double *__restrict__ uf, int nyf, int nxf)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nyf * (nxf + 1)) return;
+    int j = idx / (nxf + 1), i = idx - j * (nxf + 1);
+    uf[idx] = prol_u_at(uc, nullptr, j, i, nyf / 2, nxf / 2);
+}
+__global__ void k_prolong_v(const double *__restrict__ vc, double *__restrict__ vf, int nyf, int nxf)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (nyf + 1) * nxf) return;
+    int j = idx / nxf, i = idx - j * nxf;
+    vf[idx] = prol_v_at(vc, nullptr, j, i, nyf / 2, nxf / 2);
+}
+__global__ void k_prolong_H(const double *__restrict__ hc, double *__restrict__ hf, int nyf, int nxf)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nyf * nxf) return;
+    int j = idx / nxf, i = idx - j * nxf;
+    hf[idx] = prol_H_at(hc, nullptr, j, i, nyf / 2, nxf / 2);
+}
+
+// ---------------------------------------------------------------------------
+// FASCD element-wise steps (glide/solver.py:71-157), fused
+// ---------------------------------------------------------------------------
+// w = U, chi = gamma - H
+__global__ void k_fas_begin(size_t nt, size_t nuv, double *__restrict__ w, const double *__restrict__ U,
+                            double *__restrict__ chi, const double *__restrict__ gamma)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt) return;
+    double x = U[k];
+    w[k] = x;
+    if (k >= nuv) chi[k - nuv] = gamma[k - nuv] - x;
+}
+__global__ void k_fill(size_t n, double *__restrict__ a, double val)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = val;
+}
+__global__ void k_copy(size_t n, double *__restrict__ dst, const double *__restrict__ src)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+// gamma = w_H + chi, or (pre-smooth) gamma = w_H + ((-chi_c[inject]) + chi)
+__global__ void k_gamma(int ny, int nx, double *__restrict__ gamma, const double *__restrict__ wH,
+                        const double *__restrict__ chi, const double *__restrict__ chi_c)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ny * nx) return;
+    double ph;
+    if (chi_c) {
+        int j = idx / nx, i = idx - j * nx;
+        double inj = -1.0 * chi_c[(size_t)(j / 2) * (nx / 2) + i / 2];
+        ph = inj + chi[idx];
+    } else {
+        ph = chi[idx];
+    }
+    gamma[idx] = wH[idx] + ph;
+}
+// U = w + (P(U_c - w_c) + (U - w)): bilinear for u, v and injection for H
+// (glide/solver.py:131-140)
+__global__ void k_fas_correct(int ny, int nx, double *__restrict__ U, const double *__restrict__ w,
+                              const double *__restrict__ Uc, const double *__restrict__ wc)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
+    if (k >= nu + nv + nc) return;
+    const int nyc = ny / 2, nxc = nx / 2;
+    const size_t nuc = n_u(nyc, nxc), nvc = n_v(nyc, nxc);
+    double z;
+    if (k < nu) {
+        int j = (int)(k / (nx + 1)), i = (int)(k - (size_t)j * (nx + 1));
+        z = prol_u_at(Uc, wc, j, i, nyc, nxc);
+    } else if (k < nu + nv) {
+        size_t kk = k - nu;
+        int j = (int)(kk / nx), i = (int)(kk - (size_t)j * nx);
+        z = prol_v_at(Uc + nuc, wc + nuc, j, i, nyc, nxc);
+    } else {
+        size_t kk = k - nu - nv;
+        int j = (int)(kk / nx), i = (int)(kk - (size_t)j * nx);
+        size_t kc = nuc + nvc + (size_t)(j / 2) * nxc + i / 2;
+        z = Uc[kc] - wc[kc];
+    }
+    double y = U[k] - w[k];
+    z += y;
+    U[k] = w[k] + z;
+}
+
+// f_H = H_prev / dt + smb (glide/physics.py:173)
+__global__ void k_forcing(size_t nc, double *__restrict__ fH, const double *__restrict__ H_prev,
+                          const double *__restrict__ smb, double dt)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nc) fH[k] = H_prev[k] / dt + smb[k];
+}
+
+// ||(r_u, r_v, FB(r_H, H - gamma))||^2: per-block partial sums (fixed tree order)
+__global__ __launch_bounds__(256) void k_norm_partial(int ny, int nx, const double *__restrict__ r,
+                                                      const double *__restrict__ U,
+                                                      const double *__restrict__ gamma, double *__restrict__ part)
+{
+    __shared__ double s[256];
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx), nt = nu + nv + nc;
+    double acc = 0.0;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < nt; k += (size_t)gridDim.x * 256) {
+        double x;
+        if (k < nu + nv) {
+            x = r[k];
+        } else {
+            double a = r[k], b = U[k] - gamma[k - nu - nv];
+            x = a + b - sqrt(a * a + b * b);
+        }
+        acc += x * x;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+__global__ __launch_bounds__(256) void k_norm_final(int n, const double *__restrict__ part, double *__restrict__ out)
+{
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) acc += part[k];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sqrt(s[0]);
+}
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+namespace {
+
+int launch_residual(const GlideGeom &g, const Params &p, double *r, const double *U, const double *f, const double *sub,
+                    const double *bed, const double *B, const double *beta, const double *mask, const double *gamma,
+                    hipStream_t st)
+{
+    hipLaunchKernelGGL(k_residual, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, r, U, f, sub, bed, B, beta,
+                       mask, gamma);
+    return check_launch("residual");
+}
+
+inline int nbx_of(int nx) { return nx / RB + 1; }
+inline int nby_of(int ny) { return ny / RB + 1; }
+
+int launch_sweeps(const GlideGeom &g, const Params &p, double *U, const double *f, const double *bed, const double *B,
+                  const double *beta, const double *gamma, const unsigned char *blk, int n_sweeps, int n_newton,
+                  double *scratch, hipStream_t st)
+{
+    const int nbx = nbx_of(g.nx);
+    for (int s = 0; s < n_sweeps; s++) {
+        if (blk) {
+            hipLaunchKernelGGL(k_vanka<true>, cell_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B, beta,
+                               gamma, blk, nbx, n_newton, scratch);
+            hipLaunchKernelGGL(k_vanka_apply<true>, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g.ny, g.nx, U,
+                               scratch, blk, nbx);
+        } else {
+            hipLaunchKernelGGL(k_vanka<false>, cell_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B, beta,
+                               gamma, blk, nbx, n_newton, scratch);
+            hipLaunchKernelGGL(k_vanka_apply<false>, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g.ny, g.nx, U,
+                               scratch, blk, nbx);
+        }
+    }
+    return check_launch("vanka sweep");
+}
+
+int launch_activation(const GlideGeom &g, const double *r, const double *U, const double *gamma, unsigned char *blk,
+                      hipStream_t st)
+{
+    hipLaunchKernelGGL(k_local_activation, dim3(nbx_of(g.nx), nby_of(g.ny)), dim3(16, 16), 0, st, g.ny, g.nx, r, U,
+                       gamma, blk);
+    return check_launch("local activation");
+}
+
+int restrict_state(const GlideLevel &F, const GlideLevel &C, const double *src, double *dst, double *dst2,
+                   hipStream_t st)
+{
+    const int nyc = C.g.ny, nxc = C.g.nx, nyf = F.g.ny, nxf = F.g.nx;
+    const size_t nuf = n_u(nyf, nxf), nvf = n_v(nyf, nxf), nuc = n_u(nyc, nxc), nvc = n_v(nyc, nxc);
+    hipLaunchKernelGGL(k_restrict_u, dim3(blocks1d(nuc)), dim3(256), 0, st, src, dst, dst2, nyc, nxc);
+    hipLaunchKernelGGL(k_restrict_v, dim3(blocks1d(nvc)), dim3(256), 0, st, src + nuf, dst + nuc,
+                       dst2 ? dst2 + nuc : nullptr, nyc, nxc);
+    hipLaunchKernelGGL(k_restrict_cell, dim3(blocks1d(n_c(nyc, nxc))), dim3(256), 0, st, src + nuf + nvf,
+                       dst + nuc + nvc, dst2 ? dst2 + nuc + nvc : nullptr, nyc, nxc);
+    return check_launch("restrict state");
+}
+
+int restrict_cell(const double *hf, double *hc, int nyc, int nxc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_restrict_cell, dim3(blocks1d(n_c(nyc, nxc))), dim3(256), 0, st, hf, hc, nullptr, nyc, nxc);
+    return check_launch("restrict cell");
+}
+
+int fill(double *a, size_t n, double v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill, dim3(blocks1d(n)), dim3(256), 0, st, n, a, v);
+    return check_launch("fill");
+}
+
+#define TRY(x)                    \
+    do {                          \
+        int _rc = (x);            \
+        if (_rc != GLIDE_OK) return _rc; \
+    } while (0)
+
+GlideSchedule default_schedule() { return GlideSchedule{10, 10, 200, 10, 10, 10, 10, 100}; }
+
+int check_levels(const GlideLevel *L, int n)
+{
+    if (!L) return fail(GLIDE_E_NULL, "levels is NULL");
+    if (n < 1 || n > 16) return fail(GLIDE_E_SIZE, "level count out of range");
+    for (int k = 0; k < n; k++) {
+        TRY(check_geom(&L[k].g));
+        if (!L[k].U || !L[k].f || !L[k].r || !L[k].w || !L[k].z || !L[k].bed || !L[k].B || !L[k].beta ||
+            !L[k].gamma || !L[k].chi || !L[k].mask || !L[k].scratch || !L[k].active)
+            return fail(GLIDE_E_NULL, "level " + std::to_string(k) + " has a NULL forward buffer");
+        if (k > 0 && (L[k].g.ny * 2 != L[k - 1].g.ny || L[k].g.nx * 2 != L[k - 1].g.nx))
+            return fail(GLIDE_E_SIZE, "level " + std::to_string(k) + " is not a 2x coarsening of its parent");
+    }
+    return GLIDE_OK;
+}
+
+// glide/solver.py:71-157
+int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const GlideSchedule &sc, hipStream_t st)
+{
+    const GlideLevel &A = L[k];
+    const int ny = A.g.ny, nx = A.g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx), nt = nu + nv + nc;
+    const Params p = to_params(A.p);
+    double *wH = A.w + nu + nv;
+    if (finest) {
+        hipLaunchKernelGGL(k_fas_begin, dim3(blocks1d(nt)), dim3(256), 0, st, nt, nu + nv, A.w, A.U, A.chi, A.gamma);
+        TRY(check_launch("fas begin"));
+    }
+    if (k == n - 1) {
+        hipLaunchKernelGGL(k_gamma, dim3(blocks1d(nc)), dim3(256), 0, st, ny, nx, A.gamma, wH, A.chi, nullptr);
+        TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, nullptr, sc.coarse, sc.n_newton, A.scratch, st));
+        return fill(A.gamma, nc, thklim, st);
+    }
+    const GlideLevel &C = L[k + 1];
+    const int nyc = C.g.ny, nxc = C.g.nx;
+    const Params pc = to_params(C.p);
+    hipLaunchKernelGGL(k_restrict_max, dim3(blocks1d(n_c(nyc, nxc))), dim3(256), 0, st, A.chi, C.chi, nyc, nxc);
+    hipLaunchKernelGGL(k_gamma, dim3(blocks1d(nc)), dim3(256), 0, st, ny, nx, A.gamma, wH, A.chi, C.chi);
+    TRY(check_launch("pre gamma"));
+    TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, nullptr, sc.pre, sc.n_newton, A.scratch, st));
+    TRY(fill(A.gamma, nc, thklim, st));
+    TRY(restrict_state(A, C, A.U, C.U, C.w, st));
+    TRY(launch_residual(A.g, p, A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma, st));
+    TRY(restrict_state(A, C, A.r, C.r, nullptr, st));
+    // f_c = F_c(U_c) - r_c
+    TRY(launch_residual(C.g, pc, C.f, C.U, nullptr, C.r, C.bed, C.B, C.beta, C.mask, C.gamma, st));
+    TRY(fascd(L, n, k + 1, false, thklim, sc, st));
+    hipLaunchKernelGGL(k_fas_correct, dim3(blocks1d(nt)), dim3(256), 0, st, ny, nx, A.U, A.w, C.U, C.w);
+    hipLaunchKernelGGL(k_gamma, dim3(blocks1d(nc)), dim3(256), 0, st, ny, nx, A.gamma, wH, A.chi, nullptr);
+    TRY(check_launch("fas correct"));
+    TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, nullptr, sc.post, sc.n_newton, A.scratch, st));
+    for (int pass = 0; pass < sc.local_passes; pass++) {
+        TRY(launch_residual(A.g, p, A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma, st));
+        TRY(launch_activation(A.g, A.r, A.U, A.gamma, A.active, st));
+        TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, A.active, sc.local_sweeps, sc.n_newton,
+                          A.scratch, st));
+    }
+    return fill(A.gamma, nc, thklim, st);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *glide_last_error(void) { return g_last_error.c_str(); }
+const char *glide_version(void) { return "glide_amd 0.1 (gfx950, fp64)"; }
+
+int glide_residual(const GlideGeom *g, const GlideParams *p, double *r, const double *U, const double *f,
+                   const double *sub, const double *bed, const double *B, const double *beta, const double *mask,
+                   const double *gamma, void *stream)
+{
+    TRY(check_geom(g));
+    if (!p || !r || !U || !bed || !B || !beta || !mask || !gamma) return fail(GLIDE_E_NULL, "residual: NULL argument");
+    return launch_residual(*g, to_params(*p), r, U, f, sub, bed, B, beta, mask, gamma, S(stream));
+}
+
+int glide_vanka_sweep(const GlideGeom *g, const GlideParams *p, double *U, const double *f, const double *bed,
+                      const double *B, const double *beta, const double *gamma, const unsigned char *active,
+                      int n_sweeps, int n_newton, double *scratch, void *stream)
+{
+    TRY(check_geom(g));
+    if (!p || !U || !f || !bed || !B || !beta || !gamma || !scratch) return fail(GLIDE_E_NULL, "vanka: NULL argument");
+    if (n_sweeps < 0 || n_newton < 0) return fail(GLIDE_E_ARG, "vanka: negative count");
+    return launch_sweeps(*g, to_params(*p), U, f, bed, B, beta, gamma, active, n_sweeps, n_newton, scratch, S(stream));
+}
+
+
+int glide_vanka_block(const GlideGeom *g, const GlideParams *p, const double *U, const double *f, const double *bed,
+                      const double *B, const double *beta, const double *gamma, int k_updates, double *out,
+                      void *stream)
+{
+    TRY(check_geom(g));
+    if (!p || !U || !bed || !B || !beta || !gamma || !out) return fail(GLIDE_E_NULL, "vanka_block: NULL argument");
+    if (k_updates < 0) return fail(GLIDE_E_ARG, "vanka_block: negative count");
+    hipLaunchKernelGGL(k_vanka_block, cell_grid(g->ny, g->nx), dim3(TX, TY), 0, S(stream), *g, to_params(*p), U, f,
+                       bed, B, beta, gamma, k_updates, out);
+    return check_launch("vanka_block");
+}
+
+int glide_local_activation(const GlideGeom *g, const double *r, const double *U, const double *gamma,
+                           unsigned char *active, void *stream)
+{
+    TRY(check_geom(g));
+    if (!r || !U || !gamma || !active) return fail(GLIDE_E_NULL, "activation: NULL argument");
+    return launch_activation(*g, r, U, gamma, active, S(stream));
+}
+
+int glide_restrict_u(const double *uf, double *uc, int nyc, int nxc, void *stream)
+{
+    if (!uf || !uc) return fail(GLIDE_E_NULL, "restrict_u: NULL");
+    if (nyc < 1 || nxc < 1) return fail(GLIDE_E_SIZE, "restrict_u: size");
+    hipLaunchKernelGGL(k_restrict_u, dim3(blocks1d(n_u(nyc, nxc))), dim3(256), 0, S(stream), uf, uc, nullptr, nyc, nxc);
+    return check_launch("restrict_u");
+}
+int glide_restrict_v(const double *vf, double *vc, int nyc, int nxc, void *stream)
+{
+    if (!vf || !vc) return fail(GLIDE_E_NULL, "restrict_v: NULL");
+    if (nyc < 1 || nxc < 1) return fail(GLIDE_E_SIZE, "restrict_v: size");
+    hipLaunchKernelGGL(k_restrict_v, dim3(blocks1d(n_v(nyc, nxc))), dim3(256), 0, S(stream), vf, vc, nullptr, nyc, nxc);
+    return check_launch("restrict_v");
+}
+int glide_restrict_cell(const double *hf, double *hc, int nyc, int nxc, void *stream)
+{
+    if (!hf || !hc) return fail(GLIDE_E_NULL, "restrict_cell: NULL");
+    if (nyc < 1 || nxc < 1) return fail(GLIDE_E_SIZE, "restrict_cell: size");
+    return restrict_cell(hf, hc, nyc, nxc, S(stream));
+}
+int glide_restrict_max(const double *hf, double *hc, int nyc, int nxc, void *stream)
+{
+    if (!hf || !hc) return fail(GLIDE_E_NULL, "restrict_max: NULL");
+    if (nyc < 1 || nxc < 1) return fail(GLIDE_E_SIZE, "restrict_max: size");
+    hipLaunchKernelGGL(k_restrict_max, dim3(blocks1d(n_c(nyc, nxc))), dim3(256), 0, S(stream), hf, hc, nyc, nxc);
+    return check_launch("restrict_max");
+}
+int glide_prolong_inject(const double *hc, double *hf, int nyf, int nxf, double sign, void *stream)
+{
+    if (!hc || !hf) return fail(GLIDE_E_NULL, "prolong_inject: NULL");
+    if (nyf < 2 || nxf < 2 || (nyf & 1) || (nxf & 1)) return fail(GLIDE_E_SIZE, "prolong_inject: size");
+    hipLaunchKernelGGL(k_prolong_inject, dim3(blocks1d(n_c(nyf, nxf))), dim3(256), 0, S(stream), hc, hf, nyf, nxf, sign);
+    return check_launch("prolong_inject");
+}
+int glide_prolong_u(const double *uc, double *uf, int nyf, int nxf, void *stream)
+{
+    if (!uc || !uf) return fail(GLIDE_E_NULL, "prolong_u: NULL");
+    if (nyf < 2 || nxf < 2 || (nyf & 1) || (nxf & 1)) return fail(GLIDE_E_SIZE, "prolong_u: size");
+    hipLaunchKernelGGL(k_prolong_u, dim3(blocks1d(n_u(nyf, nxf))), dim3(256), 0, S(stream), uc, uf, nyf, nxf);
+    return check_launch("prolong_u");
+}
+int glide_prolong_v(const double *vc, double *vf, int nyf, int nxf, void *stream)
+{
+    if (!vc || !vf) return fail(GLIDE_E_NULL, "prolong_v: NULL");
+    if (nyf < 2 || nxf < 2 || (nyf & 1) || (nxf & 1)) return fail(GLIDE_E_SIZE, "prolong_v: size");
+    hipLaunchKernelGGL(k_prolong_v, dim3(blocks1d(n_v(nyf, nxf))), dim3(256), 0, S(stream), vc, vf, nyf, nxf);
+    return check_launch("prolong_v");
+}
+int glide_prolong_H(const double *hc, double *hf, int nyf, int nxf, void *stream)
+{
+    if (!hc || !hf) return fail(GLIDE_E_NULL, "prolong_H: NULL");
+    if (nyf < 2 || nxf < 2 || (nyf & 1) || (nxf & 1)) return fail(GLIDE_E_SIZE, "prolong_H: size");
+    hipLaunchKernelGGL(k_prolong_H, dim3(blocks1d(n_c(nyf, nxf))), dim3(256), 0, S(stream), hc, hf, nyf, nxf);
+    return check_launch("prolong_H");
+}
+
+int glide_restrict_parameters(const GlideLevel *L, int n, void *stream)
+{
+    TRY(check_levels(L, n));
+    hipStream_t st = S(stream);
+    for (int k = 0; k + 1 < n; k++) {
+        int nyc = L[k + 1].g.ny, nxc = L[k + 1].g.nx;
+        TRY(restrict_cell(L[k].bed, L[k + 1].bed, nyc, nxc, st));
+        TRY(restrict_cell(L[k].B, L[k + 1].B, nyc, nxc, st));
+        TRY(restrict_cell(L[k].beta, L[k + 1].beta, nyc, nxc, st));
+        if (L[k].H_prev && L[k + 1].H_prev) TRY(restrict_cell(L[k].H_prev, L[k + 1].H_prev, nyc, nxc, st));
+        if (L[k].smb && L[k + 1].smb) TRY(restrict_cell(L[k].smb, L[k + 1].smb, nyc, nxc, st));
+    }
+    return GLIDE_OK;
+}
+
+int glide_propagate_geometry(const GlideLevel *L, int n, double thklim, void *stream)
+{
+    TRY(check_levels(L, n));
+    hipStream_t st = S(stream);
+    for (int k = 0; k + 1 < n; k++) {
+        const GlideLevel &F = L[k], &C = L[k + 1];
+        int nyc = C.g.ny, nxc = C.g.nx;
+        const size_t offf = n_u(F.g.ny, F.g.nx) + n_v(F.g.ny, F.g.nx), offc = n_u(nyc, nxc) + n_v(nyc, nxc);
+        TRY(restrict_cell(F.bed, C.bed, nyc, nxc, st));
+        TRY(restrict_cell(F.U + offf, C.U + offc, nyc, nxc, st));
+        if (F.H_prev && C.H_prev) TRY(restrict_cell(F.H_prev, C.H_prev, nyc, nxc, st));
+        TRY(fill(C.gamma, n_c(nyc, nxc), thklim, st));
+    }
+    return GLIDE_OK;
+}
+
+int glide_fascd_vcycle(const GlideLevel *L, int n, double thklim, const GlideSchedule *sched, void *stream)
+{
+    TRY(check_levels(L, n));
+    GlideSchedule sc = sched ? *sched : default_schedule();
+    return fascd(L, n, 0, true, thklim, sc, S(stream));
+}
+
+int glide_forward_setup(GlideLevel *L, int n, double dt, void *stream)
+{
+    TRY(check_levels(L, n));
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(GLIDE_E_ARG, "dt must be positive");
+    if (!L[0].H_prev || !L[0].smb) return fail(GLIDE_E_NULL, "finest level needs H_prev and smb");
+    for (int k = 0; k < n; k++) L[k].g.dt = dt;
+    const size_t nc = n_c(L[0].g.ny, L[0].g.nx);
+    double *fH = L[0].f + n_u(L[0].g.ny, L[0].g.nx) + n_v(L[0].g.ny, L[0].g.nx);
+    hipLaunchKernelGGL(k_forcing, dim3(blocks1d(nc)), dim3(256), 0, S(stream), nc, fH, L[0].H_prev, L[0].smb, dt);
+    return check_launch("forcing");
+}
+
+int glide_residual_norm(const GlideLevel *level, double *out_host, void *stream)
+{
+    if (!level || !out_host) return fail(GLIDE_E_NULL, "residual_norm: NULL");
+    TRY(check_geom(&level->g));
+    const GlideLevel &A = *level;
+    hipStream_t st = S(stream);
+    TRY(launch_residual(A.g, to_params(A.p), A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma, st));
+    // partial sums go to the scratch buffer (5*nc doubles >= 1024 + 1)
+    const int nblk = 1024;
+    if (5 * n_c(A.g.ny, A.g.nx) < (size_t)nblk + 1) return fail(GLIDE_E_SIZE, "residual_norm: level too small");
+    hipLaunchKernelGGL(k_norm_partial, dim3(nblk), dim3(256), 0, st, A.g.ny, A.g.nx, A.r, A.U, A.gamma, A.scratch);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, st, nblk, A.scratch, A.scratch + nblk);
+    TRY(check_launch("residual_norm"));
+    hipError_t e = hipMemcpyAsync(out_host, A.scratch + nblk, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return fail((int)e, hipGetErrorString(e));
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail((int)e, hipGetErrorString(e));
+    return GLIDE_OK;
+}
+
+}  // extern "C"

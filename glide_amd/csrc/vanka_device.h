@@ -1,0 +1,228 @@
+// vanka_device.h - the local 5x5 block of a cell and its solve, FP64.
+//
+// Unknowns of cell (i, j): (u_l, u_r, v_t, v_b, H_c) = (u[i,j], u[i,j+1],
+// v[i,j], v[i+1,j], H[i,j]); neighbours are read from the level state and the
+// cell viscosities are frozen at the sweep's input state.  The assembly
+// follows glide/cuda/vanka.cu:56-411 (forward) and :485-780 (adjoint) term
+// by term: the block keeps only the terms the reference keeps (the shear
+// cross-couplings between u rows and v unknowns are not part of it).
+#pragma once
+
+#include "ssa_device.h"
+
+namespace glide {
+
+// 5x5 Doolittle LU without pivoting, then forward/back substitution
+// (common.cu:132-175).
+__host__ __device__ __forceinline__ void lu5_solve(const double *A, const double *b, double *x)
+{
+    double LU[5][5];
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+#pragma unroll
+        for (int j = 0; j < 5; j++) LU[i][j] = A[i * 5 + j];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        double inv = 1.0 / LU[k][k];
+#pragma unroll
+        for (int i = k + 1; i < 5; i++) {
+            LU[i][k] *= inv;
+#pragma unroll
+            for (int j = k + 1; j < 5; j++) LU[i][j] -= LU[i][k] * LU[k][j];
+        }
+    }
+    double y[5];
+    y[0] = b[0];
+    y[1] = b[1] - LU[1][0] * y[0];
+    y[2] = b[2] - LU[2][0] * y[0] - LU[2][1] * y[1];
+    y[3] = b[3] - LU[3][0] * y[0] - LU[3][1] * y[1] - LU[3][2] * y[2];
+    y[4] = b[4] - LU[4][0] * y[0] - LU[4][1] * y[1] - LU[4][2] * y[2] - LU[4][3] * y[3];
+    x[4] = y[4] / LU[4][4];
+    x[3] = (y[3] - LU[3][4] * x[4]) / LU[3][3];
+    x[2] = (y[2] - LU[2][3] * x[3] - LU[2][4] * x[4]) / LU[2][2];
+    x[1] = (y[1] - LU[1][2] * x[2] - LU[1][3] * x[3] - LU[1][4] * x[4]) / LU[1][1];
+    x[0] = (y[0] - LU[0][1] * x[1] - LU[0][2] * x[2] - LU[0][3] * x[3] - LU[0][4] * x[4]) / LU[0][0];
+}
+
+// Neighbourhood of one cell that stays fixed during its Newton loop.
+struct CellNbhd {
+    // viscosities: centre, left, right, top, bottom, and the four diagonals
+    double e_c, e_l, e_r, e_t, e_b, e_tl, e_tr, e_bl, e_br;
+    // thickness of the 8 neighbours
+    double H_l, H_r, H_t, H_b, H_tl, H_tr, H_bl, H_br;
+    double bed_c, bed_l, bed_r, bed_t, bed_b;
+    double beta_c, beta_l, beta_r, beta_t, beta_b;
+    // neighbouring velocities
+    double u_ll, u_rr, u_tl, u_tr, u_bl, u_br;   // u[i,j-1], u[i,j+2], u[i-1,j], u[i-1,j+1], u[i+1,j], u[i+1,j+1]
+    double v_lt, v_lb, v_rt, v_rb, v_tt, v_bb;   // v[i,j-1], v[i+1,j-1], v[i,j+1], v[i+1,j+1], v[i-1,j], v[i+2,j]
+    double f0, f1, f2, f3, f4;                   // right-hand side at the 5 dofs
+};
+
+// MODE 0: forward Vanka (residual, active-set test, diagonal shift)
+// MODE 1: adjoint block (no residual, mass row replaced where adj_masked)
+template <int MODE>
+__host__ __device__ __forceinline__ void vanka_assemble(const CellNbhd &N, const Params &p, int i, int j, int ny, int nx,
+                                               double dx_inv, double dt, double u_l, double u_r, double v_t,
+                                               double v_b, double H_c, double thklim, bool adj_masked,
+                                               double J[25], double r[5])
+{
+#pragma unroll
+    for (int k = 0; k < 25; k++) J[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) r[k] = 0.0;
+    if (MODE == 0) {
+        r[0] -= N.f0;
+        r[1] -= N.f1;
+        r[2] -= N.f2;
+        r[3] -= N.f3;
+        r[4] -= N.f4;
+    }
+    {
+        J[24] = 1.0 / dt;
+        r[4] += H_c / dt;
+        Calv jc = calving(H_c, N.bed_c, p);
+        J[24] -= jc.d_H;
+        r[4] -= jc.res;
+        Flux q = flux_x(u_l, N.H_l, H_c, j, nx);
+        J[20] -= q.d_u * dx_inv;
+        J[24] -= q.d_H_r * dx_inv;
+        r[4] -= q.res * dx_inv;
+        q = flux_x(u_r, H_c, N.H_r, j + 1, nx);
+        J[21] += q.d_u * dx_inv;
+        J[24] += q.d_H_l * dx_inv;
+        r[4] += q.res * dx_inv;
+        q = flux_y(v_t, N.H_t, H_c, i, ny);
+        J[22] += q.d_u * dx_inv;
+        J[24] += q.d_H_r * dx_inv;
+        r[4] += q.res * dx_inv;
+        q = flux_y(v_b, H_c, N.H_b, i + 1, ny);
+        J[23] -= q.d_u * dx_inv;
+        J[24] -= q.d_H_l * dx_inv;
+        r[4] -= q.res * dx_inv;
+        bool constrained = (MODE == 0) ? ((H_c - dt * r[4]) <= thklim) : adj_masked;
+        if (constrained) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) J[20 + k] = 0.0;
+            J[24] = 1.0;
+            if (MODE == 0) r[4] = H_c - thklim;
+        }
+    }
+    {
+        double eH = H_c * N.e_c;
+        SigN s = sigma_xx(u_l, u_r, v_t, v_b, eH, dx_inv, j, nx);
+        r[0] += s.res * dx_inv;
+        J[0] += s.d_u_l * dx_inv; J[1] += s.d_u_r * dx_inv; J[2] += s.d_v_t * dx_inv;
+        J[3] += s.d_v_b * dx_inv; J[4] += s.d_eta_H * N.e_c * dx_inv;
+        r[1] -= s.res * dx_inv;
+        J[5] -= s.d_u_l * dx_inv; J[6] -= s.d_u_r * dx_inv; J[7] -= s.d_v_t * dx_inv;
+        J[8] -= s.d_v_b * dx_inv; J[9] -= s.d_eta_H * N.e_c * dx_inv;
+        SigN t = sigma_yy(u_l, u_r, v_t, v_b, eH, dx_inv, i, ny);
+        r[2] -= t.res * dx_inv;
+        J[10] -= t.d_u_l * dx_inv; J[11] -= t.d_u_r * dx_inv; J[12] -= t.d_v_t * dx_inv;
+        J[13] -= t.d_v_b * dx_inv; J[14] -= t.d_eta_H * N.e_c * dx_inv;
+        r[3] += t.res * dx_inv;
+        J[15] += t.d_u_l * dx_inv; J[16] += t.d_u_r * dx_inv; J[17] += t.d_v_t * dx_inv;
+        J[18] += t.d_v_b * dx_inv; J[19] += t.d_eta_H * N.e_c * dx_inv;
+    }
+    {
+        SigN s = sigma_xx(N.u_ll, u_l, N.v_lt, N.v_lb, N.e_l * N.H_l, dx_inv, j - 1, nx);
+        r[0] -= s.res * dx_inv;
+        J[0] -= s.d_u_r * dx_inv;
+    }
+    {
+        SigN s = sigma_xx(u_r, N.u_rr, N.v_rt, N.v_rb, N.e_r * N.H_r, dx_inv, j + 1, nx);
+        r[1] += s.res * dx_inv;
+        J[6] += s.d_u_l * dx_inv;
+    }
+    {
+        SigN s = sigma_yy(N.u_tl, N.u_tr, N.v_tt, v_t, N.e_t * N.H_t, dx_inv, i - 1, ny);
+        r[2] += s.res * dx_inv;
+        J[12] += s.d_v_b * dx_inv;
+    }
+    {
+        SigN s = sigma_yy(N.u_bl, N.u_br, v_b, N.v_bb, N.e_b * N.H_b, dx_inv, i + 1, ny);
+        r[3] -= s.res * dx_inv;
+        J[18] -= s.d_v_t * dx_inv;
+    }
+    {   // top-left vertex (i, j)
+        double eH = etaH_vertex(N.e_tl, N.e_t, N.e_l, N.e_c, N.H_tl, N.H_t, N.H_l, H_c);
+        double dH = 0.25 * N.e_c;
+        SigS s = sigma_xy(N.u_tl, u_l, N.v_lt, v_t, eH, dx_inv, i, j, ny, nx);
+        r[0] += s.res * dx_inv;
+        J[0] += s.d_u_b * dx_inv;
+        J[4] += s.d_eta_H * dH * dx_inv;
+        r[2] -= s.res * dx_inv;
+        J[12] -= s.d_v_r * dx_inv;
+        J[14] -= s.d_eta_H * dH * dx_inv;
+    }
+    {   // top-right vertex (i, j+1)
+        double eH = etaH_vertex(N.e_t, N.e_tr, N.e_c, N.e_r, N.H_t, N.H_tr, H_c, N.H_r);
+        double dH = 0.25 * N.e_c;
+        SigS s = sigma_xy(N.u_tr, u_r, v_t, N.v_rt, eH, dx_inv, i, j + 1, ny, nx);
+        r[1] += s.res * dx_inv;
+        J[6] += s.d_u_b * dx_inv;
+        J[9] += s.d_eta_H * dH * dx_inv;
+        r[2] += s.res * dx_inv;
+        J[12] += s.d_v_l * dx_inv;
+        J[14] += s.d_eta_H * dH * dx_inv;
+    }
+    {   // bottom-left vertex (i+1, j)
+        double eH = etaH_vertex(N.e_l, N.e_c, N.e_bl, N.e_b, N.H_l, H_c, N.H_bl, N.H_b);
+        double dH = 0.25 * N.e_c;
+        SigS s = sigma_xy(u_l, N.u_bl, N.v_lb, v_b, eH, dx_inv, i + 1, j, ny, nx);
+        r[0] -= s.res * dx_inv;
+        J[0] -= s.d_u_t * dx_inv;
+        J[4] -= s.d_eta_H * dH * dx_inv;
+        r[3] -= s.res * dx_inv;
+        J[18] -= s.d_v_r * dx_inv;
+        J[19] -= s.d_eta_H * dH * dx_inv;
+    }
+    {   // bottom-right vertex (i+1, j+1)
+        double eH = etaH_vertex(N.e_c, N.e_r, N.e_b, N.e_br, H_c, N.H_r, N.H_b, N.H_br);
+        double dH = 0.25 * N.e_c;
+        SigS s = sigma_xy(u_r, N.u_br, v_b, N.v_rb, eH, dx_inv, i + 1, j + 1, ny, nx);
+        r[1] -= s.res * dx_inv;
+        J[6] -= s.d_u_t * dx_inv;
+        J[9] -= s.d_eta_H * dH * dx_inv;
+        r[3] += s.res * dx_inv;
+        J[18] += s.d_v_l * dx_inv;
+        J[19] += s.d_eta_H * dH * dx_inv;
+    }
+    {
+        TauB t = tau_b(u_l, N.H_l, H_c, N.bed_l, N.bed_c, N.beta_l, N.beta_c, p);
+        r[0] += t.res; J[0] += t.d_u; J[4] += t.d_H_r;
+    }
+    {
+        TauB t = tau_b(u_r, H_c, N.H_r, N.bed_c, N.bed_r, N.beta_c, N.beta_r, p);
+        r[1] += t.res; J[6] += t.d_u; J[9] += t.d_H_l;
+    }
+    {
+        TauB t = tau_b(v_t, N.H_t, H_c, N.bed_t, N.bed_c, N.beta_t, N.beta_c, p);
+        r[2] += t.res; J[12] += t.d_u; J[14] += t.d_H_r;
+    }
+    {
+        TauB t = tau_b(v_b, H_c, N.H_b, N.bed_c, N.bed_b, N.beta_c, N.beta_b, p);
+        r[3] += t.res; J[18] += t.d_u; J[19] += t.d_H_l;
+    }
+    {
+        TauD t = tau_dx(N.H_l, H_c, N.bed_l, N.bed_c, dx_inv, j, nx);
+        r[0] -= t.res; J[4] -= t.d_H_r;
+    }
+    {
+        TauD t = tau_dx(H_c, N.H_r, N.bed_c, N.bed_r, dx_inv, j + 1, nx);
+        r[1] -= t.res; J[9] -= t.d_H_l;
+    }
+    {
+        TauD t = tau_dy(N.H_t, H_c, N.bed_t, N.bed_c, dx_inv, i, ny);
+        r[2] -= t.res; J[14] -= t.d_H_r;
+    }
+    {
+        TauD t = tau_dy(H_c, N.H_b, N.bed_c, N.bed_b, dx_inv, i + 1, ny);
+        r[3] -= t.res; J[19] -= t.d_H_l;
+    }
+    if (MODE == 0) {
+        J[0] -= 1.0; J[6] -= 1.0; J[12] -= 1.0; J[18] -= 1.0; J[24] += 1.0;
+    }
+}
+
+}  // namespace glide

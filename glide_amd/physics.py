@@ -1,0 +1,206 @@
+"""Public API: IcePhysics (counterpart of glide/physics.py:19-259) plus the
+misfit / regularisation helpers of the inversion (glide/physics.py:262-411).
+
+Every solver step runs on the GPU through the C ABI of libglide_hip.so; this
+module only orders the calls, moves inputs into the level buffers and exposes
+views of the state.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .grid import Grid, descriptors, stream_handle
+
+RHO_ICE = 917.0
+G = 9.81
+
+
+def _as_f64(x, device) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device=device, dtype=torch.float64)
+    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=device)
+
+
+class IcePhysics:
+    """Shallow-shelf ice-flow model on a MAC grid, FP64 on one MI355X.
+
+    Same constructor, set_geometry / set_parameters / forward / adjoint
+    semantics as the reference (glide/physics.py:49-242).  forward() returns
+    views (u, v, H) of the finest level's state; u and v warm-start the next
+    call.
+    """
+
+    def __init__(self, ny, nx, dx, n_levels=5, n=3.0, eps_reg=1e-5, thklim=0.1, water_drag=1e-3,
+                 calving_rate=1.0, gl_sigmoid_c=0.1, gl_derivatives=False, device=None):
+        if ny % (1 << (n_levels - 1)) or nx % (1 << (n_levels - 1)):
+            raise ValueError(f"grid {ny}x{nx} is not divisible by 2^(n_levels-1) = {1 << (n_levels - 1)}")
+        _lib.load()
+        self.ny, self.nx, self.dx, self.n_levels = int(ny), int(nx), float(dx), int(n_levels)
+        self.n, self.eps_reg, self.thklim = float(n), float(eps_reg), float(thklim)
+        self.water_drag, self.calving_rate = float(water_drag), float(calving_rate)
+        self.gl_sigmoid_c, self.gl_derivatives = float(gl_sigmoid_c), bool(gl_derivatives)
+        self.device = torch.device(device if device is not None else "cuda")
+        self.grid = Grid(ny, nx, dx, 1.0, self.device, n=n, eps_reg=eps_reg, water_drag=water_drag,
+                         calving_rate=calving_rate, gl_sigmoid_c=gl_sigmoid_c, gl_derivatives=gl_derivatives)
+        self.grids = [self.grid]
+        for _ in range(n_levels - 1):
+            self.grids.append(self.grids[-1].spawn_child())
+        self.schedule = None          # None = the reference's sweep counts
+        self._desc = None
+        self.last_history: list[float] = []
+
+    # ------------------------------------------------------------ helpers
+    def _levels(self):
+        if self._desc is None:
+            self._desc = descriptors(self.grids)
+        for k, g in enumerate(self.grids):
+            self._desc[k].g.dt = g.dt
+        return self._desc
+
+    def _stream(self):
+        return ctypes.c_void_p(stream_handle(self.device))
+
+    def set_schedule(self, pre=10, post=10, coarse=200, local_passes=10, local_sweeps=10, n_newton=10,
+                     adj_sweeps=10, adj_coarse=100):
+        self.schedule = _lib.GlideSchedule(pre, post, coarse, local_passes, local_sweeps, n_newton, adj_sweeps,
+                                           adj_coarse)
+
+    # ------------------------------------------------------------ inputs
+    def set_geometry(self, bed, thickness):
+        """glide/physics.py:82-99"""
+        g = self.grid
+        g.bed.copy_(_as_f64(bed, self.device).reshape(g.ny, g.nx))
+        g.H.copy_(_as_f64(thickness, self.device).reshape(g.ny, g.nx))
+        g.H_prev.copy_(g.H)
+        g.gamma.fill_(self.thklim)
+        _lib.call("glide_propagate_geometry", self._levels(), self.n_levels, self.thklim, self._stream())
+
+    def set_parameters(self, B=None, beta=None, smb=None):
+        """glide/physics.py:101-129"""
+        g = self.grid
+        if B is not None:
+            Bt = _as_f64(B, self.device)
+            if Bt.ndim == 0:
+                g.B.fill_(float(Bt))
+            else:
+                g.B.copy_(Bt.reshape(g.ny, g.nx))
+        if beta is not None:
+            g.beta.copy_(_as_f64(beta, self.device).reshape(g.ny, g.nx))
+        if smb is not None:
+            st = _as_f64(smb, self.device)
+            if st.ndim == 0:
+                g.smb.fill_(float(st))
+            else:
+                g.smb.copy_(st.reshape(g.ny, g.nx))
+        _lib.call("glide_restrict_parameters", self._levels(), self.n_levels, self._stream())
+
+    # ------------------------------------------------------------ solve
+    def residual_norm(self) -> float:
+        out = ctypes.c_double(0.0)
+        _lib.call("glide_residual_norm", ctypes.byref(self._levels()[0]), ctypes.byref(out), self._stream())
+        return out.value
+
+    def vcycle(self):
+        sched = ctypes.byref(self.schedule) if self.schedule is not None else None
+        _lib.call("glide_fascd_vcycle", self._levels(), self.n_levels, self.thklim, sched, self._stream())
+
+    def forward(self, dt, n_vcycles=3, verbose=False, update_geometry=True, tol=None, max_vcycles=None):
+        """One implicit time step (glide/physics.py:141-209).
+
+        Runs ``n_vcycles`` FASCD V-cycles; with ``tol`` it instead runs until
+        the relative residual ||(r_u, r_v, FB(r_H, H - gamma))|| / r0 drops
+        to ``tol`` (at most ``max_vcycles``).  The relative residual after
+        each cycle is kept in ``last_history`` when it was computed.
+        """
+        for g in self.grids:
+            g.dt = float(dt)
+        levels = self._levels()
+        _lib.call("glide_forward_setup", levels, self.n_levels, float(dt), self._stream())
+        track = verbose or tol is not None
+        self.last_history = []
+        r0 = self.residual_norm() if track else 0.0
+        if verbose:
+            print(f"  Initial: |r| = {r0:.2e}")
+        n_max = n_vcycles if tol is None else (max_vcycles or max(n_vcycles, 50))
+        for it in range(n_max):
+            self.vcycle()
+            if track:
+                rel = self.residual_norm() / r0 if r0 > 0 else 0.0
+                self.last_history.append(rel)
+                if verbose:
+                    print(f"  V-cycle {it}: |r|/|r0| = {rel:.2e}")
+                if tol is not None and rel <= tol:
+                    break
+        if update_geometry:
+            self.grid.H_prev.copy_(self.grid.H)
+        return self.grid.u, self.grid.v, self.grid.H
+
+    # ------------------------------------------------------------ misc API
+    def reset_solution(self):
+        """glide/physics.py:244-248"""
+        self.grid.u.zero_()
+        self.grid.v.zero_()
+        self.grid.H.copy_(self.grid.H_prev)
+
+    def get_surface(self):
+        """glide/physics.py:250-253"""
+        base = torch.maximum(self.grid.bed, -RHO_ICE / 1000.0 * self.grid.H)
+        return self.grid.H + base
+
+    def get_velocities_cell_centered(self):
+        """glide/physics.py:255-259"""
+        u_c = 0.5 * (self.grid.u[:, 1:] + self.grid.u[:, :-1])
+        v_c = 0.5 * (self.grid.v[1:] + self.grid.v[:-1])
+        return u_c, v_c
+
+
+# ---------------------------------------------------------------- losses
+def _nan_zero(d: torch.Tensor) -> torch.Tensor:
+    return torch.where(torch.isnan(d), torch.zeros_like(d), d)
+
+
+def huber_loss(u, v, u_obs, v_obs, eps=10.0):
+    """glide/physics.py:262-287"""
+    du, dv = _nan_zero(u - u_obs), _nan_zero(v - v_obs)
+    return torch.sqrt(du ** 2 + eps).mean() + torch.sqrt(dv ** 2 + eps).mean()
+
+
+def huber_grad(u, v, u_obs, v_obs, eps=10.0):
+    """glide/physics.py:290-318"""
+    du, dv = _nan_zero(u - u_obs), _nan_zero(v - v_obs)
+    n = u.numel()
+    return du / torch.sqrt(du ** 2 + eps) / n, dv / torch.sqrt(dv ** 2 + eps) / n
+
+
+def abs_loss(u, v, u_obs, v_obs):
+    """glide/physics.py:320-349"""
+    du = _nan_zero(u.double() - u_obs.double())
+    dv = _nan_zero(v.double() - v_obs.double())
+    return du.abs().mean() + dv.abs().mean()
+
+
+def abs_grad(u, v, u_obs, v_obs, eps=10.0):
+    """glide/physics.py:352-379"""
+    du, dv = _nan_zero(u - u_obs), _nan_zero(v - v_obs)
+    n = u.numel()
+    return torch.sign(du) / n, torch.sign(dv) / n
+
+
+def tikhonov_regularization(field: torch.Tensor):
+    """0.5 * (sum dx^2 + sum dy^2) of neighbour differences and its gradient
+    with one-sided ends (glide/physics.py:382-411)."""
+    dxf = field[:, 1:] - field[:, :-1]
+    dyf = field[1:, :] - field[:-1, :]
+    loss = 0.5 * ((dxf ** 2).sum() + (dyf ** 2).sum())
+    grad = torch.zeros_like(field)
+    grad[:, 1:-1] -= field[:, 2:] - 2 * field[:, 1:-1] + field[:, :-2]
+    grad[:, 0] -= field[:, 1] - field[:, 0]
+    grad[:, -1] -= field[:, -2] - field[:, -1]
+    grad[1:-1, :] -= field[2:, :] - 2 * field[1:-1, :] + field[:-2, :]
+    grad[0, :] -= field[1, :] - field[0, :]
+    grad[-1, :] -= field[-2, :] - field[-1, :]
+    return float(loss), grad

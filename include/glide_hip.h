@@ -1,0 +1,198 @@
+/*
+ * glide_hip.h - C ABI of the MI355X-native GLIDE SSA / FASCD solver.
+ *
+ * All arrays are FP64, row-major MAC-grid fields in device (HBM) memory:
+ *   u  : ny x (nx+1)  x-velocity on the left face of each cell
+ *   v  : (ny+1) x nx  y-velocity on the top face of each cell
+ *   H  : ny x nx      thickness (and every other cell field)
+ * A "state vector" is [u | v | H] contiguous (n_total = nu + nv + nc
+ * values), the layout of glide/grid.py:79-165.
+ *
+ * Conventions for every entry point:
+ *   - plain device pointers and sizes, no framework types;
+ *   - nothing is allocated: scratch buffers are caller-provided;
+ *   - work is enqueued on the caller's stream (hipStream_t passed as void*,
+ *     NULL = the default stream); no entry point synchronises the host
+ *     except glide_residual_norm, which reads back one scalar;
+ *   - return 0 on success, a negative GLIDE_E_* code for invalid arguments,
+ *     or a positive hipError_t from the runtime; glide_last_error() gives a
+ *     message for the last failure on the calling thread.
+ */
+#ifndef GLIDE_HIP_H
+#define GLIDE_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLIDE_OK 0
+#define GLIDE_E_NULL (-1)      /* required pointer is NULL */
+#define GLIDE_E_SIZE (-2)      /* bad grid size / level count */
+#define GLIDE_E_ARG (-3)       /* other invalid argument */
+#define GLIDE_E_NONFINITE (-4) /* non-finite input detected */
+
+/* Physical parameters: replaces the PhysicsParams struct
+ * (glide/cuda/common.cu:240-247) packed by make_physics_params
+ * (glide/kernels.py:10-49); FP64 here. */
+typedef struct GlideParams {
+    double n;            /* Glen exponent */
+    double eps_reg;      /* strain-rate regularisation */
+    double water_drag;   /* drag of floating ice */
+    double calving_rate; /* calving rate of floating ice */
+    double gl_sigmoid_c; /* grounding-line sigmoid sharpness */
+    int gl_derivatives;  /* include d(sigmoid)/dH in Jacobians */
+} GlideParams;
+
+/* One grid level's geometry: the (ny, nx, dx, dt) kernel arguments of
+ * glide/grid.py:195-203. */
+typedef struct GlideGeom {
+    int ny, nx;
+    double dx, dt;
+} GlideGeom;
+
+/* All buffers of one multigrid level (glide/grid.py:79-158). Adjoint
+ * buffers may be NULL when no adjoint is run. */
+typedef struct GlideLevel {
+    GlideGeom g;
+    GlideParams p;
+    double *U, *f, *r, *w, *z;                   /* n_total each */
+    double *Lambda, *Lambda_out, *f_adj, *r_adj; /* n_total each (adjoint) */
+    double *H_prev, *bed, *B, *beta, *smb, *gamma, *chi, *mask; /* nc each */
+    double *scratch;      /* 5 * nc doubles (Vanka per-cell updates) */
+    unsigned char *active;/* (ny/14+1)*(nx/14+1) bytes: local-smoother blocks */
+} GlideLevel;
+
+/* Sweep counts of one FASCD / adjoint V-cycle (hard-coded in the reference:
+ * glide/solver.py:96,109,144,147-155,177,183,219; glide/grid.py:305). */
+typedef struct GlideSchedule {
+    int pre, post, coarse;          /* 10, 10, 200 */
+    int local_passes, local_sweeps; /* 10, 10 */
+    int n_newton;                   /* 10 */
+    int adj_sweeps, adj_coarse;     /* 10, 100 */
+} GlideSchedule;
+
+/* ---- level operators ------------------------------------------------- */
+
+/* r = F(U) - f on one level (f == NULL gives F(U), i.e. compute_F);
+ * when sub != NULL, r = F(U) - f - sub (the FAS right-hand side
+ * f_c = F_c(U_c) - r_c in one pass).
+ * Replaces kernel compute_residual (glide/cuda/residuals.cu:6-292) as
+ * launched by Grid.compute_residual / compute_F (glide/grid.py:190-223). */
+int glide_residual(const GlideGeom *g, const GlideParams *p, double *r, const double *U,
+                   const double *f, const double *sub, const double *bed, const double *B,
+                   const double *beta, const double *mask, const double *gamma, void *stream);
+
+/* j = J(U) dU (exact forward-mode linearisation of the residual).
+ * Replaces kernel compute_jvp (glide/cuda/residuals.cu:299-579),
+ * Grid.compute_jvp (glide/grid.py:225-238). */
+int glide_jvp(const GlideGeom *g, const GlideParams *p, double *j, const double *U,
+              const double *dU, const double *bed, const double *B, const double *beta,
+              const double *mask, const double *gamma, void *stream);
+
+/* l = J(U)^T lambda, gather form (deterministic, no atomics).
+ * Replaces kernel compute_vjp (glide/cuda/residuals.cu:586-1023),
+ * Grid.compute_vjp (glide/grid.py:240-254, including its l.fill(0)). */
+int glide_vjp(const GlideGeom *g, const GlideParams *p, double *l, const double *U,
+              const double *lam, const double *bed, const double *B, const double *beta,
+              const double *mask, const double *gamma, void *stream);
+
+/* n_sweeps forward Vanka sweeps, each = per-cell damped Newton on the 5
+ * local unknowns with eta frozen, then U += 0.5 * delta.  active == NULL
+ * smooths every cell; otherwise only cells whose 14x14 reference block has
+ * active[block] != 0 (see glide_local_activation).  scratch: 5*nc doubles.
+ * Replaces kernels vanka_smooth / vanka_smooth_local
+ * (glide/cuda/vanka.cu:3-439, 844-1295) and the host loops
+ * Grid.vanka_sweep / vanka_sweep_local (glide/grid.py:256-317). */
+int glide_vanka_sweep(const GlideGeom *g, const GlideParams *p, double *U, const double *f,
+                      const double *bed, const double *B, const double *beta,
+                      const double *gamma, const unsigned char *active, int n_sweeps,
+                      int n_newton, double *scratch, void *stream);
+
+/* Local 5x5 block (J, r) of every cell after k_updates damped Newton
+ * updates of its 5 unknowns (out: 30 doubles per cell, J row-major then r).
+ * Exposes the per-cell assembly of glide/cuda/vanka.cu:56-411 for testing. */
+int glide_vanka_block(const GlideGeom *g, const GlideParams *p, const double *U, const double *f,
+                      const double *bed, const double *B, const double *beta, const double *gamma,
+                      int k_updates, double *out, void *stream);
+
+/* Block activation of the local smoother: error = |r_u right| + |r_u left|
+ * + |r_v bottom| + |r_v top| + |FB(r_H, H - gamma)| > 0.01 anywhere in a
+ * 14x14 block's 16x16 launch footprint.  Replaces the error-mask driver
+ * (glide/solver.py:147-154) plus the block test of vanka_smooth_local
+ * (glide/cuda/vanka.cu:870-888). */
+int glide_local_activation(const GlideGeom *g, const double *r, const double *U,
+                           const double *gamma, unsigned char *active, void *stream);
+
+/* One colour of the adjoint red-black Vanka smoother: lam_out = lam +
+ * omega * (1/2 faces, 1 H) * J_loc^-T r_adj on cells with (i+j)%2 == color.
+ * lam_out must hold a copy of lam on entry.  Replaces kernel
+ * vanka_smooth_adjoint (glide/cuda/vanka.cu:441-840) / Grid.vanka_smooth_adjoint
+ * (glide/grid.py:286-303). */
+int glide_vanka_adjoint_pass(const GlideGeom *g, const GlideParams *p, double *lam_out,
+                             const double *lam, const double *mask, const double *r_adj,
+                             const double *U, const double *bed, const double *B,
+                             const double *beta, const double *gamma, int color,
+                             double omega, void *stream);
+
+/* grad = lambda . dR/dbeta.  reference_edges != 0 reproduces the
+ * reference's skipped edge terms (glide/cuda/grad.cu:52-53,70-71);
+ * 0 gives the exact transpose.  Replaces kernel compute_grad_beta
+ * (glide/cuda/grad.cu:1-74) / Grid.compute_grad_beta (glide/grid.py:336-355). */
+int glide_grad_beta(const GlideGeom *g, const GlideParams *p, double *grad, const double *U,
+                    const double *lam, const double *bed, const double *beta,
+                    int reference_edges, void *stream);
+
+/* Transfer operators (glide/cuda/utility.cu; launchers glide/kernels.py:90-236).
+ * Sizes are the coarse (restrict) or fine (prolong) grid. */
+int glide_restrict_u(const double *uf, double *uc, int nyc, int nxc, void *stream);     /* restrict_u :1-19 */
+int glide_restrict_v(const double *vf, double *vc, int nyc, int nxc, void *stream);     /* restrict_v :125-143 */
+int glide_restrict_cell(const double *hf, double *hc, int nyc, int nxc, void *stream);  /* restrict_cell_centered :56-75 */
+int glide_restrict_max(const double *hf, double *hc, int nyc, int nxc, void *stream);   /* restrict_max_pool :77-101 */
+int glide_prolong_inject(const double *hc, double *hf, int nyf, int nxf, double sign,
+                         void *stream);                                                 /* prolongate_cell_centered :104-123 */
+int glide_prolong_u(const double *uc, double *uf, int nyf, int nxf, void *stream);      /* prolongate_u_smooth :180-232 */
+int glide_prolong_v(const double *vc, double *vf, int nyf, int nxf, void *stream);      /* prolongate_v_smooth :235-287 */
+int glide_prolong_H(const double *hc, double *hf, int nyf, int nxf, void *stream);      /* prolongate_H_smooth :289-337 */
+
+/* ---- hierarchy drivers ------------------------------------------------ */
+
+/* Restrict bed, B, beta, H_prev, smb down the hierarchy
+ * (glide/solver.py:53-68, restrict_parameters_to_hierarchy). */
+int glide_restrict_parameters(const GlideLevel *levels, int n_levels, void *stream);
+
+/* Restrict bed, H (into U), H_prev and reset gamma = thklim on the children
+ * (glide/physics.py:131-139). */
+int glide_propagate_geometry(const GlideLevel *levels, int n_levels, double thklim,
+                             void *stream);
+
+/* One FASCD V-cycle from the finest level (glide/solver.py:71-157,
+ * fascd_vcycle(grid, thklim, finest=True)).  sched == NULL uses the
+ * reference's counts. */
+int glide_fascd_vcycle(const GlideLevel *levels, int n_levels, double thklim,
+                       const GlideSchedule *sched, void *stream);
+
+/* f_H = H_prev / dt + smb on the finest level and dt on every level
+ * (glide/physics.py:166-173). */
+int glide_forward_setup(GlideLevel *levels, int n_levels, double dt, void *stream);
+
+/* ||(r_u, r_v, FB(r_H, H - gamma))||_2 of the finest level into *out_host
+ * (the verbose norm of glide/physics.py:176-203); synchronises the stream. */
+int glide_residual_norm(const GlideLevel *level, double *out_host, void *stream);
+
+/* One adjoint V-cycle (glide/solver.py:160-220). */
+int glide_adjoint_vcycle(const GlideLevel *levels, int n_levels, const GlideSchedule *sched,
+                         void *stream);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *glide_last_error(void);
+
+/* Library version string. */
+const char *glide_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
