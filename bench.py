@@ -1,0 +1,172 @@
+"""Benchmark of the forward step of the FP64 SSA / FASCD solver on one MI355X.
+
+Workload cap_512: 512 x 512 MAC grid, dx = 1500 m, 5 levels, dt = 10 a,
+3 FASCD V-cycles per step (the reference's step, glide/physics.py:141-209),
+seeded synthetic ice cap with a floating margin (glide_amd/synthetic.py).
+
+One "step" = IcePhysics.forward(dt, n_vcycles=3) on inputs already resident in
+HBM.  Prints ONE JSON line.  Besides the step time it reports:
+  - the relative residual after the timed steps and whether 1e-8 is reached;
+  - roofline: the dominant kernel (the finest-level Vanka sweep, FP64
+    arithmetic) timed with HIP events on the solver's stream, priced with the
+    algorithmic flop count per cell-iteration frozen below;
+  - stencil_roofline: the residual stencil (HBM) the same way;
+  - cpu_baseline: the CPU oracle (glide_oracle.c, OpenMP) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from glide_amd.roofline import (FLOPS_NEWTON_ITER, PEAK_FP64_TFLOPS, PEAK_HBM_GBS,  # noqa: E402
+                                residual_bytes, sweep_flops)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cap_512")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the kernel microbenchmarks (for profiling them alone)")
+    return ap.parse_args()
+
+
+def kernel_rooflines(ph, reps=20):
+    """Time the finest-level full Vanka sweep and residual with HIP events."""
+    import ctypes
+
+    import torch
+
+    from glide_amd import _lib
+
+    g = ph.grid
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    geom, prm = g.geom(), g.cparams()
+    keep = g.U.clone()
+
+    def vanka():
+        _lib.call("glide_vanka_sweep", ctypes.byref(geom), ctypes.byref(prm), ctypes.c_void_p(g.U.data_ptr()),
+                  ctypes.c_void_p(g.f.data_ptr()), ctypes.c_void_p(g.bed.data_ptr()),
+                  ctypes.c_void_p(g.B.data_ptr()), ctypes.c_void_p(g.beta.data_ptr()),
+                  ctypes.c_void_p(g.gamma.data_ptr()), None, 1, 10, ctypes.c_void_p(g.scratch.data_ptr()), sp)
+
+    def resid():
+        _lib.call("glide_residual", ctypes.byref(geom), ctypes.byref(prm), ctypes.c_void_p(g.r.data_ptr()),
+                  ctypes.c_void_p(g.U.data_ptr()), ctypes.c_void_p(g.f.data_ptr()), None,
+                  ctypes.c_void_p(g.bed.data_ptr()), ctypes.c_void_p(g.B.data_ptr()),
+                  ctypes.c_void_p(g.beta.data_ptr()), ctypes.c_void_p(g.mask.data_ptr()),
+                  ctypes.c_void_p(g.gamma.data_ptr()), sp)
+
+    out = {}
+    for name, fn in (("vanka", vanka), ("residual", resid)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        out[name] = e0.elapsed_time(e1) / reps   # ms per call
+    g.U.copy_(keep)
+    torch.cuda.synchronize()
+    nc = g.ny * g.nx
+    vanka_flops = sweep_flops(nc)
+    # the sweep time includes the small gather/apply pass; it is charged to the sweep
+    vanka_tf = vanka_flops / (out["vanka"] * 1e-3) / 1e12
+    res_bytes = residual_bytes(g.ny, g.nx)
+    res_gbs = res_bytes / (out["residual"] * 1e-3) / 1e9
+    return {
+        "roofline": {"kernel": "k_vanka<false> + k_vanka_apply (finest full sweep, 512^2)", "bound": "mfma",
+                     "note": "FP64 VALU arithmetic (no MFMA); peak is MI355X FP64 78.6 TFLOP/s",
+                     "achieved": round(vanka_tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(vanka_tf / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "ms_per_launch": round(out["vanka"], 4),
+                     "work": f"{nc} cells x (eta + 10 Newton x {FLOPS_NEWTON_ITER} flop)"},
+        "stencil_roofline": {"kernel": "k_residual (512^2)", "bound": "hbm", "achieved": round(res_gbs, 1),
+                             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(res_gbs / PEAK_HBM_GBS, 4),
+                             "traffic": None, "ms_per_launch": round(out["residual"], 4),
+                             "work": f"{res_bytes} algorithmic bytes"},
+    }
+
+
+def cpu_baseline(w, threads: int):
+    """The CPU oracle on one V-cycle of the second step (about 5-20 s)."""
+    from oracle import oracle as O
+
+    if threads > 0:
+        O.set_threads(threads)
+    h = O.Hierarchy(w["ny"], w["nx"], w["dx"], n_levels=w["n_levels"])
+    h.set_geometry(w["bed"], w["H"])
+    h.set_parameters(B=w["B"], beta=w["beta"], smb=w["smb"])
+    h.forward(w["dt"], n_vcycles=1)        # first step with one cycle: a warm state
+    t0 = time.perf_counter()
+    h.forward(w["dt"], n_vcycles=1)
+    dt_cycle = time.perf_counter() - t0
+    return {"value": round(dt_cycle * w["n_vcycles"] * 1e3, 1), "unit": "ms per forward step",
+            "cores": O.num_threads(), "kind": "port",
+            "sample": f"1 V-cycle of step 2 of {w['name']} timed ({dt_cycle:.1f} s), x{w['n_vcycles']} V-cycles per step"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU is out of scope)"}))
+        sys.exit(2)
+    import torch
+
+    from glide_amd import synthetic as S
+    from glide_amd.physics import IcePhysics
+
+    w = S.workload(args.workload)
+    ph = IcePhysics(w["ny"], w["nx"], w["dx"], n_levels=w["n_levels"])
+    ph.set_geometry(w["bed"], w["H"])
+    ph.set_parameters(B=w["B"], beta=w["beta"], smb=w["smb"])
+    torch.cuda.synchronize()
+    if args.kernels_only:
+        ph.forward(w["dt"], n_vcycles=w["n_vcycles"])
+        print(json.dumps(kernel_rooflines(ph)))
+        return
+    for _ in range(args.warmup):
+        ph.forward(w["dt"], n_vcycles=w["n_vcycles"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ph.forward(w["dt"], n_vcycles=w["n_vcycles"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    # relative residual reached by one more step (not timed)
+    ph.forward(w["dt"], n_vcycles=w["n_vcycles"], tol=1e-300, max_vcycles=w["n_vcycles"])
+    hist = ph.last_history
+    rl = kernel_rooflines(ph)
+    line = {
+        "metric": "ms per forward step (3 FASCD V-cycles, FP64) at 512^2; relative residual reached vs 1e-8",
+        "value": round(ms, 3), "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded ice cap, glide_amd/synthetic.py)",
+        "config": {"workload": f"{w['name']}: {w['ny']}x{w['nx']}, dx={w['dx']:.0f} m, {w['n_levels']} levels, "
+                               f"dt={w['dt']:.0f} a, {w['n_vcycles']} V-cycles/step",
+                   "global_batch": 1, "parallelism": "single GPU"},
+        "rel_residual_after_step": [float(f"{x:.3e}") for x in hist],
+        "reaches_1e-8": bool(hist and min(hist) <= 1e-8),
+        **rl,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(w, args.cpu_threads)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -74,6 +74,8 @@ SIGNATURES = {
     "glide_forward_setup": (c_int, [POINTER(GlideLevel), c_int, c_double, c_void_p]),
     "glide_residual_norm": (c_int, [POINTER(GlideLevel), POINTER(c_double), c_void_p]),
     "glide_adjoint_vcycle": (c_int, [POINTER(GlideLevel), c_int, POINTER(GlideSchedule), c_void_p]),
+    "glide_compute_mask": (c_int, [POINTER(GlideLevel), c_void_p]),
+    "glide_adjoint_residual": (c_int, [POINTER(GlideLevel), c_void_p]),
     "glide_last_error": (c_char_p, []),
     "glide_version": (c_char_p, []),
 }
@@ -88,6 +90,10 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing; build it with `python -m glide_amd.build`")
+    # torch carries its own HIP runtime with the same soname (libamdhip64.so.7);
+    # loading torch first makes this library bind to that one instance, so
+    # device pointers and streams from torch are valid here.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
     for name in ("glide_last_error", "glide_version"):
         fn = getattr(lib, name)

@@ -16,7 +16,7 @@
  *         2^k through the exponent field.
  *   log : x = m * 2^e with m in [sqrt(2)/2, sqrt(2)); f = m - 1 (exact),
  *         s = f / (2 + f), log(1 + f) = 2 atanh(s) written as
- *         f - (f^2/2 - s (f^2/2 + R(s^2))) with R the atanh series to s^22.
+ *         f - (f^2/2 - s (f^2/2 + Q(s^2))) with Q the atanh series to s^22.
  *   pow : exp(p * log(x)) for x > 0 (relative error ~ (|p log x| + 2) ulp).
  * Accuracy against libm is checked by the test suite (a few ulp).
  */
@@ -97,20 +97,20 @@ DM_FN double dm_log(double x)
     double f = m - 1.0;
     double s = f / (2.0 + f);
     double z = s * s;
-    double R = 0.08695652173913043;             /* 2/23 */
-    R = 0.09523809523809523 + z * R;            /* 2/21 */
-    R = 0.10526315789473684 + z * R;            /* 2/19 */
-    R = 0.11764705882352941 + z * R;            /* 2/17 */
-    R = 0.13333333333333333 + z * R;            /* 2/15 */
-    R = 0.15384615384615385 + z * R;            /* 2/13 */
-    R = 0.18181818181818182 + z * R;            /* 2/11 */
-    R = 0.2222222222222222 + z * R;             /* 2/9  */
-    R = 0.2857142857142857 + z * R;             /* 2/7  */
-    R = 0.4 + z * R;                            /* 2/5  */
-    R = 0.6666666666666666 + z * R;             /* 2/3  */
-    R = z * R;
+    double Q = 0.08695652173913043;             /* 2/23 */
+    Q = 0.09523809523809523 + z * Q;            /* 2/21 */
+    Q = 0.10526315789473684 + z * Q;            /* 2/19 */
+    Q = 0.11764705882352941 + z * Q;            /* 2/17 */
+    Q = 0.13333333333333333 + z * Q;            /* 2/15 */
+    Q = 0.15384615384615385 + z * Q;            /* 2/13 */
+    Q = 0.18181818181818182 + z * Q;            /* 2/11 */
+    Q = 0.2222222222222222 + z * Q;             /* 2/9  */
+    Q = 0.2857142857142857 + z * Q;             /* 2/7  */
+    Q = 0.4 + z * Q;                            /* 2/5  */
+    Q = 0.6666666666666666 + z * Q;             /* 2/3  */
+    Q = z * Q;
     double hfsq = 0.5 * f * f;
-    double lm = f - (hfsq - s * (hfsq + R));
+    double lm = f - (hfsq - s * (hfsq + Q));
     double ed = (double)e;
     return ed * LN2_HI + (ed * LN2_LO + lm);
 }

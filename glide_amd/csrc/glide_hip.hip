@@ -632,6 +632,434 @@ __global__ __launch_bounds__(256) void k_norm_final(int n, const double *__restr
     if (threadIdx.x == 0) out[0] = sqrt(s[0]);
 }
 
+
+// ===========================================================================
+// Linearisation: JVP and gather-form VJP
+// ===========================================================================
+__device__ __forceinline__ void fill_eta_dual_tile(Dual (*eta_s)[TX + 2], int i0, int j0, const double *__restrict__ u,
+                                                   const double *__restrict__ v, const double *__restrict__ du,
+                                                   const double *__restrict__ dv, const double *__restrict__ B,
+                                                   const Params &p, double dx, int ny, int nx)
+{
+    int tid = threadIdx.y * TX + threadIdx.x;
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        int a = k / (TX + 2), b = k - a * (TX + 2);
+        int ci = i0 - 1 + a, cj = j0 - 1 + b;
+        eta_s[a][b] = (ci >= 0 && ci < ny && cj >= 0 && cj < nx) ? eta_cell_dual(ci, cj, u, v, du, dv, B, p, dx, ny, nx)
+                                                                 : Dual{0.0, 0.0};
+    }
+}
+
+// d/dt of the u-row R_u(i, j) along (du, dv, dH); dH == NULL means 0
+// (the u-face part of glide/cuda/residuals.cu:378-499)
+template <class EF>
+__device__ __forceinline__ double jvp_u_row(int i, int j, int ny, int nx, double dx_inv, const Params &p, const EF &E,
+                                            const double *u, const double *v, const double *H, const double *du,
+                                            const double *dv, const double *dH, const double *bed,
+                                            const double *beta)
+{
+    auto Hd = [&](int a, int b) { return Dual{ld_c(H, a, b, ny, nx), dH ? ld_c(dH, a, b, ny, nx) : 0.0}; };
+    auto Ud = [&](int a, int b) { return Dual{ld_u(u, a, b, ny, nx), ld_u(du, a, b, ny, nx)}; };
+    auto Vd = [&](int a, int b) { return Dual{ld_v(v, a, b, ny, nx), ld_v(dv, a, b, ny, nx)}; };
+    auto eHc = [](Dual eta, Dual Hh) { return Dual{Hh.v * eta.v, Hh.v * eta.d + eta.v * Hh.d}; };
+    auto eHv = [](Dual a, Dual b, Dual c, Dual d, Dual Ha, Dual Hb, Dual Hc, Dual Hdd) {
+        return Dual{etaH_vertex(a.v, b.v, c.v, d.v, Ha.v, Hb.v, Hc.v, Hdd.v),
+                    0.25 * Ha.v * a.d + 0.25 * a.v * Ha.d + 0.25 * Hb.v * b.d + 0.25 * b.v * Hb.d +
+                        0.25 * Hc.v * c.d + 0.25 * c.v * Hc.d + 0.25 * Hdd.v * d.d + 0.25 * d.v * Hdd.d};
+    };
+    double d = 0.0;
+    {
+        Dual eH = eHc(E(i, j), Hd(i, j));
+        Dual ul = Ud(i, j), ur = Ud(i, j + 1), vt = Vd(i, j), vb = Vd(i + 1, j);
+        SigN s = sigma_xx(ul.v, ur.v, vt.v, vb.v, eH.v, dx_inv, j, nx);
+        d += sigN_jvp(s, ul.d, ur.d, vt.d, vb.d, eH.d) * dx_inv;
+    }
+    {
+        Dual eH = eHc(E(i, j - 1), Hd(i, j - 1));
+        Dual ul = Ud(i, j), ull = Ud(i, j - 1), vt = Vd(i, j - 1), vb = Vd(i + 1, j - 1);
+        SigN s = sigma_xx(ull.v, ul.v, vt.v, vb.v, eH.v, dx_inv, j - 1, nx);
+        d -= sigN_jvp(s, ull.d, ul.d, vt.d, vb.d, eH.d) * dx_inv;
+    }
+    {
+        Dual eH = eHv(E(i - 1, j - 1), E(i - 1, j), E(i, j - 1), E(i, j), Hd(i - 1, j - 1), Hd(i - 1, j), Hd(i, j - 1),
+                      Hd(i, j));
+        Dual ut = Ud(i - 1, j), ub = Ud(i, j), vl = Vd(i, j - 1), vr = Vd(i, j);
+        SigS s = sigma_xy(ut.v, ub.v, vl.v, vr.v, eH.v, dx_inv, i, j, ny, nx);
+        d += sigS_jvp(s, ut.d, ub.d, vl.d, vr.d, eH.d) * dx_inv;
+    }
+    {
+        Dual eH = eHv(E(i, j - 1), E(i, j), E(i + 1, j - 1), E(i + 1, j), Hd(i, j - 1), Hd(i, j), Hd(i + 1, j - 1),
+                      Hd(i + 1, j));
+        Dual ut = Ud(i, j), ub = Ud(i + 1, j), vl = Vd(i + 1, j - 1), vr = Vd(i + 1, j);
+        SigS s = sigma_xy(ut.v, ub.v, vl.v, vr.v, eH.v, dx_inv, i + 1, j, ny, nx);
+        d -= sigS_jvp(s, ut.d, ub.d, vl.d, vr.d, eH.d) * dx_inv;
+    }
+    {
+        Dual ul = Ud(i, j), Hl = Hd(i, j - 1), Hc = Hd(i, j);
+        TauB t = tau_b(ul.v, Hl.v, Hc.v, ld_c(bed, i, j - 1, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i, j - 1, ny, nx),
+                       ld_c(beta, i, j, ny, nx), p);
+        d += t.d_u * ul.d + t.d_H_l * Hl.d + t.d_H_r * Hc.d;
+    }
+    {
+        Dual Hl = Hd(i, j - 1), Hc = Hd(i, j);
+        TauD t = tau_dx(Hl.v, Hc.v, ld_c(bed, i, j - 1, ny, nx), ld_c(bed, i, j, ny, nx), dx_inv, j, nx);
+        d -= t.d_H_l * Hl.d + t.d_H_r * Hc.d;
+    }
+    return d;
+}
+
+// v-row R_v(i, j) (glide/cuda/residuals.cu:501-576)
+template <class EF>
+__device__ __forceinline__ double jvp_v_row(int i, int j, int ny, int nx, double dx_inv, const Params &p, const EF &E,
+                                            const double *u, const double *v, const double *H, const double *du,
+                                            const double *dv, const double *dH, const double *bed,
+                                            const double *beta)
+{
+    auto Hd = [&](int a, int b) { return Dual{ld_c(H, a, b, ny, nx), dH ? ld_c(dH, a, b, ny, nx) : 0.0}; };
+    auto Ud = [&](int a, int b) { return Dual{ld_u(u, a, b, ny, nx), ld_u(du, a, b, ny, nx)}; };
+    auto Vd = [&](int a, int b) { return Dual{ld_v(v, a, b, ny, nx), ld_v(dv, a, b, ny, nx)}; };
+    auto eHc = [](Dual eta, Dual Hh) { return Dual{Hh.v * eta.v, Hh.v * eta.d + eta.v * Hh.d}; };
+    auto eHv = [](Dual a, Dual b, Dual c, Dual d, Dual Ha, Dual Hb, Dual Hc, Dual Hdd) {
+        return Dual{etaH_vertex(a.v, b.v, c.v, d.v, Ha.v, Hb.v, Hc.v, Hdd.v),
+                    0.25 * Ha.v * a.d + 0.25 * a.v * Ha.d + 0.25 * Hb.v * b.d + 0.25 * b.v * Hb.d +
+                        0.25 * Hc.v * c.d + 0.25 * c.v * Hc.d + 0.25 * Hdd.v * d.d + 0.25 * d.v * Hdd.d};
+    };
+    double d = 0.0;
+    {
+        Dual eH = eHc(E(i - 1, j), Hd(i - 1, j));
+        Dual ul = Ud(i - 1, j), ur = Ud(i - 1, j + 1), vt = Vd(i - 1, j), vb = Vd(i, j);
+        SigN s = sigma_yy(ul.v, ur.v, vt.v, vb.v, eH.v, dx_inv, i - 1, ny);
+        d += sigN_jvp(s, ul.d, ur.d, vt.d, vb.d, eH.d) * dx_inv;
+    }
+    {
+        Dual eH = eHc(E(i, j), Hd(i, j));
+        Dual ul = Ud(i, j), ur = Ud(i, j + 1), vt = Vd(i, j), vb = Vd(i + 1, j);
+        SigN s = sigma_yy(ul.v, ur.v, vt.v, vb.v, eH.v, dx_inv, i, ny);
+        d -= sigN_jvp(s, ul.d, ur.d, vt.d, vb.d, eH.d) * dx_inv;
+    }
+    {
+        Dual eH = eHv(E(i - 1, j - 1), E(i - 1, j), E(i, j - 1), E(i, j), Hd(i - 1, j - 1), Hd(i - 1, j), Hd(i, j - 1),
+                      Hd(i, j));
+        Dual ut = Ud(i - 1, j), ub = Ud(i, j), vl = Vd(i, j - 1), vr = Vd(i, j);
+        SigS s = sigma_xy(ut.v, ub.v, vl.v, vr.v, eH.v, dx_inv, i, j, ny, nx);
+        d -= sigS_jvp(s, ut.d, ub.d, vl.d, vr.d, eH.d) * dx_inv;
+    }
+    {
+        Dual eH = eHv(E(i - 1, j), E(i - 1, j + 1), E(i, j), E(i, j + 1), Hd(i - 1, j), Hd(i - 1, j + 1), Hd(i, j),
+                      Hd(i, j + 1));
+        Dual ut = Ud(i - 1, j + 1), ub = Ud(i, j + 1), vl = Vd(i, j), vr = Vd(i, j + 1);
+        SigS s = sigma_xy(ut.v, ub.v, vl.v, vr.v, eH.v, dx_inv, i, j + 1, ny, nx);
+        d += sigS_jvp(s, ut.d, ub.d, vl.d, vr.d, eH.d) * dx_inv;
+    }
+    {
+        Dual vt = Vd(i, j), Ht = Hd(i - 1, j), Hc = Hd(i, j);
+        TauB t = tau_b(vt.v, Ht.v, Hc.v, ld_c(bed, i - 1, j, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i - 1, j, ny, nx),
+                       ld_c(beta, i, j, ny, nx), p);
+        d += t.d_u * vt.d + t.d_H_l * Ht.d + t.d_H_r * Hc.d;
+    }
+    {
+        Dual Ht = Hd(i - 1, j), Hc = Hd(i, j);
+        TauD t = tau_dy(Ht.v, Hc.v, ld_c(bed, i - 1, j, ny, nx), ld_c(bed, i, j, ny, nx), dx_inv, i, ny);
+        d -= t.d_H_l * Ht.d + t.d_H_r * Hc.d;
+    }
+    return d;
+}
+
+__global__ __launch_bounds__(NT) void k_jvp(GlideGeom g, Params p, double *__restrict__ jo, const double *__restrict__ U,
+                                            const double *__restrict__ dU, const double *__restrict__ bed,
+                                            const double *__restrict__ B, const double *__restrict__ beta,
+                                            const double *__restrict__ mask)
+{
+    __shared__ Dual eta_s[TY + 2][TX + 2];
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv, *du = dU, *dv = dU + nu, *dH = dU + nu + nv;
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    fill_eta_dual_tile(eta_s, i0, j0, u, v, du, dv, B, p, g.dx, ny, nx);
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    if (i > ny || j > nx) return;
+    auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+    if (i < ny && j < nx) {   // glide/cuda/residuals.cu:343-375
+        Dual H_c = {ld_c(H, i, j, ny, nx), ld_c(dH, i, j, ny, nx)};
+        double d = H_c.d / dt;
+        Calv jc = calving(H_c.v, ld_c(bed, i, j, ny, nx), p);
+        d -= jc.d_H * H_c.d;
+        double Hl = ld_c(H, i, j - 1, ny, nx), dHl = ld_c(dH, i, j - 1, ny, nx);
+        double ul = ld_u(u, i, j, ny, nx), dul = ld_u(du, i, j, ny, nx);
+        Flux q = flux_x(ul, Hl, H_c.v, j, nx);
+        d -= (q.d_u * dul + q.d_H_l * dHl + q.d_H_r * H_c.d) * dx_inv;
+        double Hr = ld_c(H, i, j + 1, ny, nx), dHr = ld_c(dH, i, j + 1, ny, nx);
+        double ur = ld_u(u, i, j + 1, ny, nx), dur = ld_u(du, i, j + 1, ny, nx);
+        q = flux_x(ur, H_c.v, Hr, j + 1, nx);
+        d += (q.d_u * dur + q.d_H_l * H_c.d + q.d_H_r * dHr) * dx_inv;
+        double Ht = ld_c(H, i - 1, j, ny, nx), dHt = ld_c(dH, i - 1, j, ny, nx);
+        double vt = ld_v(v, i, j, ny, nx), dvt = ld_v(dv, i, j, ny, nx);
+        q = flux_y(vt, Ht, H_c.v, i, ny);
+        d += (q.d_u * dvt + q.d_H_l * dHt + q.d_H_r * H_c.d) * dx_inv;
+        double Hb = ld_c(H, i + 1, j, ny, nx), dHb = ld_c(dH, i + 1, j, ny, nx);
+        double vb = ld_v(v, i + 1, j, ny, nx), dvb = ld_v(dv, i + 1, j, ny, nx);
+        q = flux_y(vb, H_c.v, Hb, i + 1, ny);
+        d -= (q.d_u * dvb + q.d_H_l * H_c.d + q.d_H_r * dHb) * dx_inv;
+        double m = ld_c(mask, i, j, ny, nx);
+        jo[nu + nv + (size_t)i * nx + j] = (1.0 - m) * d + m * H_c.d;
+    }
+    if (i < ny) jo[(size_t)i * (nx + 1) + j] = jvp_u_row(i, j, ny, nx, dx_inv, p, E, u, v, H, du, dv, dH, bed, beta);
+    if (j < nx) jo[nu + (size_t)i * nx + j] = jvp_v_row(i, j, ny, nx, dx_inv, p, E, u, v, H, du, dv, dH, bed, beta);
+}
+
+// shear-rate derivative factor 2*eps_xy at vertex (a, b), 0 on boundary vertices
+__device__ __forceinline__ double vertex_deps(int a, int b, int ny, int nx, double dx_inv, const double *u,
+                                              const double *v)
+{
+    SigS s = sigma_xy(ld_u(u, a - 1, b, ny, nx), ld_u(u, a, b, ny, nx), ld_v(v, a, b - 1, ny, nx), ld_v(v, a, b, ny, nx),
+                      0.0, dx_inv, a, b, ny, nx);
+    return s.d_eta_H;
+}
+
+// l = J^T lambda in gather form; out = base - l when base != NULL
+// (the adjoint residual r_adj = f_adj - J^T lambda, glide/grid.py:322-324)
+__global__ __launch_bounds__(NT) void k_vjp(GlideGeom g, Params p, double *__restrict__ out, const double *__restrict__ base,
+                                            const double *__restrict__ U, const double *__restrict__ lam,
+                                            const double *__restrict__ bed, const double *__restrict__ B,
+                                            const double *__restrict__ beta, const double *__restrict__ mask)
+{
+    __shared__ Dual eta_s[TY + 2][TX + 2];
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv, *lu = lam, *lv = lam + nu, *lH = lam + nu + nv;
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    fill_eta_dual_tile(eta_s, i0, j0, u, v, lu, lv, B, p, g.dx, ny, nx);
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    if (i > ny || j > nx) return;
+    auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+    // lambda_H of the mass row of cell (a, b) with the active-set mask applied
+    auto lm = [&](int a, int b) { return ld_c(lH, a, b, ny, nx) * (1.0 - ld_c(mask, a, b, ny, nx)); };
+    auto emit = [&](size_t k, double val) { out[k] = base ? base[k] - val : val; };
+
+    if (i < ny) {   // u face (i, j)
+        Flux q = flux_x(ld_u(u, i, j, ny, nx), ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx), j, nx);
+        double acc = 0.0;
+        if (j > 0) acc += lm(i, j - 1) * q.d_u * dx_inv;
+        if (j < nx) acc += -lm(i, j) * q.d_u * dx_inv;
+        acc += jvp_u_row(i, j, ny, nx, dx_inv, p, E, u, v, H, lu, lv, (const double *)nullptr, bed, beta);
+        emit((size_t)i * (nx + 1) + j, acc);
+    }
+    if (j < nx) {   // v face (i, j)
+        Flux q = flux_y(ld_v(v, i, j, ny, nx), ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j, ny, nx), i, ny);
+        double acc = 0.0;
+        if (i > 0) acc += -lm(i - 1, j) * q.d_u * dx_inv;
+        if (i < ny) acc += lm(i, j) * q.d_u * dx_inv;
+        acc += jvp_v_row(i, j, ny, nx, dx_inv, p, E, u, v, H, lu, lv, (const double *)nullptr, bed, beta);
+        emit(nu + (size_t)i * nx + j, acc);
+    }
+    if (i < ny && j < nx) {   // cell (i, j)
+        const double Hc = ld_c(H, i, j, ny, nx), eta = E(i, j).v;
+        const double Hl = ld_c(H, i, j - 1, ny, nx), Hr = ld_c(H, i, j + 1, ny, nx);
+        const double Ht = ld_c(H, i - 1, j, ny, nx), Hb = ld_c(H, i + 1, j, ny, nx);
+        const double lc = lm(i, j);
+        Flux qxl = flux_x(ld_u(u, i, j, ny, nx), Hl, Hc, j, nx);
+        Flux qxr = flux_x(ld_u(u, i, j + 1, ny, nx), Hc, Hr, j + 1, nx);
+        Flux qyt = flux_y(ld_v(v, i, j, ny, nx), Ht, Hc, i, ny);
+        Flux qyb = flux_y(ld_v(v, i + 1, j, ny, nx), Hc, Hb, i + 1, ny);
+        Calv jc = calving(Hc, ld_c(bed, i, j, ny, nx), p);
+        // own mass row (residuals.cu:648-690)
+        double acc = lc / dt;
+        acc += -lc * jc.d_H;
+        acc += -lc * qxl.d_H_r * dx_inv;
+        acc += lc * qxr.d_H_l * dx_inv;
+        acc += lc * qyt.d_H_r * dx_inv;
+        acc += -lc * qyb.d_H_l * dx_inv;
+        acc += ld_c(lH, i, j, ny, nx) * ld_c(mask, i, j, ny, nx);
+        // neighbours' mass rows through the shared faces
+        if (j > 0) acc += lm(i, j - 1) * qxl.d_H_r * dx_inv;
+        if (j < nx - 1) acc += -lm(i, j + 1) * qxr.d_H_l * dx_inv;
+        if (i > 0) acc += -lm(i - 1, j) * qyt.d_H_r * dx_inv;
+        if (i < ny - 1) acc += lm(i + 1, j) * qyb.d_H_l * dx_inv;
+        // momentum rows through eta*H of this cell: normal stresses
+        const double ul = ld_u(u, i, j, ny, nx), ur = ld_u(u, i, j + 1, ny, nx);
+        const double vt = ld_v(v, i, j, ny, nx), vb = ld_v(v, i + 1, j, ny, nx);
+        SigN sx = sigma_xx(ul, ur, vt, vb, 0.0, dx_inv, j, nx);
+        SigN sy = sigma_yy(ul, ur, vt, vb, 0.0, dx_inv, i, ny);
+        const double lu_l = ld_u(lu, i, j, ny, nx), lu_r = ld_u(lu, i, j + 1, ny, nx);
+        const double lv_t = ld_v(lv, i, j, ny, nx), lv_b = ld_v(lv, i + 1, j, ny, nx);
+        acc += lu_l * sx.d_eta_H * eta * dx_inv;
+        acc += -lu_r * sx.d_eta_H * eta * dx_inv;
+        acc += -lv_t * sy.d_eta_H * eta * dx_inv;
+        acc += lv_b * sy.d_eta_H * eta * dx_inv;
+        // shear stresses of the four corner vertices
+        const double q4 = 0.25 * eta;
+        for (int c = 0; c < 4; c++) {
+            const int a = i + (c >> 1), b = j + (c & 1);
+            const double dv_ = vertex_deps(a, b, ny, nx, dx_inv, u, v);
+            const double w = ld_u(lu, a, b, ny, nx) - ld_u(lu, a - 1, b, ny, nx) - ld_v(lv, a, b, ny, nx) +
+                             ld_v(lv, a, b - 1, ny, nx);
+            acc += w * dv_ * q4 * dx_inv;
+        }
+        // basal drag (grounding-line derivatives) and driving stress on the four faces
+        {
+            TauB t = tau_b(ul, Hl, Hc, ld_c(bed, i, j - 1, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i, j - 1, ny, nx),
+                           ld_c(beta, i, j, ny, nx), p);
+            TauD d = tau_dx(Hl, Hc, ld_c(bed, i, j - 1, ny, nx), ld_c(bed, i, j, ny, nx), dx_inv, j, nx);
+            acc += lu_l * t.d_H_r;
+            acc += -lu_l * d.d_H_r;
+        }
+        {
+            TauB t = tau_b(ur, Hc, Hr, ld_c(bed, i, j, ny, nx), ld_c(bed, i, j + 1, ny, nx), ld_c(beta, i, j, ny, nx),
+                           ld_c(beta, i, j + 1, ny, nx), p);
+            TauD d = tau_dx(Hc, Hr, ld_c(bed, i, j, ny, nx), ld_c(bed, i, j + 1, ny, nx), dx_inv, j + 1, nx);
+            acc += lu_r * t.d_H_l;
+            acc += -lu_r * d.d_H_l;
+        }
+        {
+            TauB t = tau_b(vt, Ht, Hc, ld_c(bed, i - 1, j, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i - 1, j, ny, nx),
+                           ld_c(beta, i, j, ny, nx), p);
+            TauD d = tau_dy(Ht, Hc, ld_c(bed, i - 1, j, ny, nx), ld_c(bed, i, j, ny, nx), dx_inv, i, ny);
+            acc += lv_t * t.d_H_r;
+            acc += -lv_t * d.d_H_r;
+        }
+        {
+            TauB t = tau_b(vb, Hc, Hb, ld_c(bed, i, j, ny, nx), ld_c(bed, i + 1, j, ny, nx), ld_c(beta, i, j, ny, nx),
+                           ld_c(beta, i + 1, j, ny, nx), p);
+            TauD d = tau_dy(Hc, Hb, ld_c(bed, i, j, ny, nx), ld_c(bed, i + 1, j, ny, nx), dx_inv, i + 1, ny);
+            acc += lv_b * t.d_H_l;
+            acc += -lv_b * d.d_H_l;
+        }
+        emit(nu + nv + (size_t)i * nx + j, acc);
+    }
+}
+
+// Adjoint red-black Vanka pass, in place: cells of one colour own disjoint
+// dofs, so lam is read and written by one lane per dof
+// (glide/cuda/vanka.cu:441-840 with Lambda_out = Lambda, grid.py:291-303)
+__global__ __launch_bounds__(NT) void k_vanka_adj(GlideGeom g, Params p, double *lam_out, const double *lam,
+                                                  const double *__restrict__ mask, const double *__restrict__ r_adj,
+                                                  const double *__restrict__ U, const double *__restrict__ bed,
+                                                  const double *__restrict__ B, const double *__restrict__ beta,
+                                                  const double *__restrict__ gamma, int color, double omega)
+{
+    __shared__ double eta_s[TY + 2][TX + 2];
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv;
+    const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    fill_eta_tile(eta_s, i0, j0, u, v, B, p, g.dx, ny, nx);
+    __syncthreads();
+    if (i >= ny || j >= nx || ((i + j) & 1) != color) return;
+    auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+    CellNbhd N = load_nbhd(i, j, ny, nx, nu, nv, u, v, H, (const double *)nullptr, bed, beta, E);
+    double J[25], r[5], JT[25], rhs[5], d[5];
+    const bool masked = ld_c(mask, i, j, ny, nx) > 0.5;
+    vanka_assemble<1>(N, p, i, j, ny, nx, dx_inv, dt, ld_u(u, i, j, ny, nx), ld_u(u, i, j + 1, ny, nx),
+                      ld_v(v, i, j, ny, nx), ld_v(v, i + 1, j, ny, nx), ld_c(H, i, j, ny, nx), ld_c(gamma, i, j, ny, nx),
+                      masked, J, r);
+#pragma unroll
+    for (int a = 0; a < 5; a++)
+#pragma unroll
+        for (int b = 0; b < 5; b++) JT[a * 5 + b] = J[b * 5 + a];
+    const size_t ku0 = (size_t)i * (nx + 1) + j, ku1 = ku0 + 1;
+    const size_t kv0 = nu + (size_t)i * nx + j, kv1 = nu + (size_t)(i + 1) * nx + j;
+    const size_t kh = nu + nv + (size_t)i * nx + j;
+    rhs[0] = r_adj[ku0];
+    rhs[1] = r_adj[ku1];
+    rhs[2] = r_adj[kv0];
+    rhs[3] = r_adj[kv1];
+    rhs[4] = r_adj[kh];
+    lu5_solve(JT, rhs, d);
+    const double l0 = lam[ku0], l1 = lam[ku1], l2 = lam[kv0], l3 = lam[kv1], l4 = lam[kh];
+    lam_out[ku0] = l0 + 0.5 * omega * d[0];
+    lam_out[ku1] = l1 + 0.5 * omega * d[1];
+    lam_out[kv0] = l2 + 0.5 * omega * d[2];
+    lam_out[kv1] = l3 + 0.5 * omega * d[3];
+    lam_out[kh] = l4 + omega * d[4];
+}
+
+// grad_beta = lambda . dR/dbeta, gather form in the oracle's summation order
+// (glide/cuda/grad.cu:1-74); ref_edges reproduces the reference's skipped
+// edge terms, otherwise boundary faces' ghost terms go to the clamped cell.
+__global__ void k_grad_beta(GlideGeom g, Params p, double *__restrict__ grad, const double *__restrict__ U,
+                            const double *__restrict__ lam, const double *__restrict__ bed,
+                            const double *__restrict__ beta, int ref_edges)
+{
+    const int ny = g.ny, nx = g.nx;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ny * nx) return;
+    const int i = idx / nx, j = idx - i * nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv, *lu = lam, *lv = lam + nu;
+    auto tbx = [&](int a, int b) {   // drag on u face (a, b)
+        return tau_b(ld_u(u, a, b, ny, nx), ld_c(H, a, b - 1, ny, nx), ld_c(H, a, b, ny, nx), ld_c(bed, a, b - 1, ny, nx),
+                     ld_c(bed, a, b, ny, nx), ld_c(beta, a, b - 1, ny, nx), ld_c(beta, a, b, ny, nx), p);
+    };
+    auto tby = [&](int a, int b) {   // drag on v face (a, b)
+        return tau_b(ld_v(v, a, b, ny, nx), ld_c(H, a - 1, b, ny, nx), ld_c(H, a, b, ny, nx), ld_c(bed, a - 1, b, ny, nx),
+                     ld_c(bed, a, b, ny, nx), ld_c(beta, a - 1, b, ny, nx), ld_c(beta, a, b, ny, nx), p);
+    };
+    double acc = 0.0;
+    {   // u face (i, j): this cell is its right cell
+        TauB t = tbx(i, j);
+        double l = lu[(size_t)i * (nx + 1) + j];
+        if (!ref_edges && j == 0) acc += l * t.d_beta_l;
+        if (!ref_edges || j < nx - 1) acc += l * t.d_beta_r;
+    }
+    {   // v face (i, j): this cell is its bottom cell
+        TauB t = tby(i, j);
+        double l = lv[(size_t)i * nx + j];
+        if (!ref_edges && i == 0) acc += l * t.d_beta_l;
+        if (!ref_edges || i < ny - 1) acc += l * t.d_beta_r;
+    }
+    {   // u face (i, j+1): left cell
+        TauB t = tbx(i, j + 1);
+        double l = lu[(size_t)i * (nx + 1) + j + 1];
+        acc += l * t.d_beta_l;
+        if (!ref_edges && j + 1 == nx) acc += l * t.d_beta_r;
+    }
+    {   // v face (i+1, j): top cell
+        TauB t = tby(i + 1, j);
+        double l = lv[(size_t)(i + 1) * nx + j];
+        acc += l * t.d_beta_l;
+        if (!ref_edges && i + 1 == ny) acc += l * t.d_beta_r;
+    }
+    grad[idx] = acc;
+}
+
+// mask = (H <= gamma + 0.1) (glide/grid.py:332-334)
+__global__ void k_compute_mask(size_t nc, double *__restrict__ mask, const double *__restrict__ H,
+                               const double *__restrict__ gamma)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nc) mask[k] = (H[k] <= gamma[k] + 0.1) ? 1.0 : 0.0;
+}
+
+// lam += P(lam_c): bilinear for all three fields (glide/solver.py:207-215)
+__global__ void k_adj_correct(int ny, int nx, double *__restrict__ lam, const double *__restrict__ lamc)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
+    if (k >= nu + nv + nc) return;
+    const int nyc = ny / 2, nxc = nx / 2;
+    const size_t nuc = n_u(nyc, nxc), nvc = n_v(nyc, nxc);
+    double z;
+    if (k < nu) {
+        int j = (int)(k / (nx + 1)), i = (int)(k - (size_t)j * (nx + 1));
+        z = prol_u_at(lamc, nullptr, j, i, nyc, nxc);
+    } else if (k < nu + nv) {
+        size_t kk = k - nu;
+        int j = (int)(kk / nx), i = (int)(kk - (size_t)j * nx);
+        z = prol_v_at(lamc + nuc, nullptr, j, i, nyc, nxc);
+    } else {
+        size_t kk = k - nu - nv;
+        int j = (int)(kk / nx), i = (int)(kk - (size_t)j * nx);
+        z = prol_H_at(lamc + nuc + nvc, nullptr, j, i, nyc, nxc);
+    }
+    lam[k] = lam[k] + z;
+}
+
 // ===========================================================================
 // Host side
 // ===========================================================================
@@ -770,6 +1198,77 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
     return fill(A.gamma, nc, thklim, st);
 }
 
+
+int launch_vjp(const GlideGeom &g, const Params &p, double *out, const double *base, const double *U, const double *lam,
+               const double *bed, const double *B, const double *beta, const double *mask, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_vjp, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, out, base, U, lam, bed, B, beta, mask);
+    return check_launch("vjp");
+}
+
+int compute_mask(const GlideLevel &A, hipStream_t st)
+{
+    const size_t nc = n_c(A.g.ny, A.g.nx), off = n_u(A.g.ny, A.g.nx) + n_v(A.g.ny, A.g.nx);
+    hipLaunchKernelGGL(k_compute_mask, dim3(blocks1d(nc)), dim3(256), 0, st, nc, A.mask, A.U + off, A.gamma);
+    return check_launch("compute_mask");
+}
+
+// glide/grid.py:319-330: per sweep, for each colour: r_adj = f_adj - J^T lam; smooth
+int adjoint_sweeps(const GlideLevel &A, int n, double omega, hipStream_t st)
+{
+    const Params p = to_params(A.p);
+    for (int s = 0; s < n; s++) {
+        for (int color = 0; color < 2; color++) {
+            TRY(launch_vjp(A.g, p, A.r_adj, A.f_adj, A.U, A.Lambda, A.bed, A.B, A.beta, A.mask, st));
+            hipLaunchKernelGGL(k_vanka_adj, cell_grid(A.g.ny, A.g.nx), dim3(TX, TY), 0, st, A.g, p, A.Lambda, A.Lambda,
+                               A.mask, A.r_adj, A.U, A.bed, A.B, A.beta, A.gamma, color, omega);
+            TRY(check_launch("vanka adjoint"));
+        }
+    }
+    return GLIDE_OK;
+}
+
+int check_adjoint_levels(const GlideLevel *L, int n)
+{
+    TRY(check_levels(L, n));
+    for (int k = 0; k < n; k++)
+        if (!L[k].Lambda || !L[k].f_adj || !L[k].r_adj)
+            return fail(GLIDE_E_NULL, "level " + std::to_string(k) + " has a NULL adjoint buffer");
+    return GLIDE_OK;
+}
+
+// glide/solver.py:160-220
+int adjoint_vc(const GlideLevel *L, int n, int k, const GlideSchedule &sc, hipStream_t st)
+{
+    const GlideLevel &A = L[k];
+    const size_t nc = n_c(A.g.ny, A.g.nx), nt = n_t(A.g.ny, A.g.nx);
+    if (k == n - 1) {
+        TRY(compute_mask(A, st));
+        TRY(adjoint_sweeps(A, sc.adj_coarse, 1.0, st));
+        return fill(A.mask, nc, 0.0, st);
+    }
+    const GlideLevel &C = L[k + 1];
+    const int nyc = C.g.ny, nxc = C.g.nx;
+    TRY(compute_mask(A, st));
+    TRY(adjoint_sweeps(A, sc.adj_sweeps, 1.0, st));
+    TRY(fill(A.mask, nc, 0.0, st));
+    TRY(launch_vjp(A.g, to_params(A.p), A.r_adj, A.f_adj, A.U, A.Lambda, A.bed, A.B, A.beta, A.mask, st));
+    // restrict_solution(adjoint=True) writes the child's lambda, which is reset below: skipped
+    TRY(restrict_cell(A.bed, C.bed, nyc, nxc, st));
+    TRY(restrict_cell(A.B, C.B, nyc, nxc, st));
+    TRY(restrict_cell(A.beta, C.beta, nyc, nxc, st));
+    if (A.H_prev && C.H_prev) TRY(restrict_cell(A.H_prev, C.H_prev, nyc, nxc, st));
+    if (A.smb && C.smb) TRY(restrict_cell(A.smb, C.smb, nyc, nxc, st));
+    TRY(restrict_state(A, C, A.r_adj, C.f_adj, nullptr, st));
+    TRY(fill(C.Lambda, n_t(nyc, nxc), 0.0, st));
+    if (C.Lambda_out) TRY(fill(C.Lambda_out, n_t(nyc, nxc), 0.0, st));
+    TRY(adjoint_vc(L, n, k + 1, sc, st));
+    hipLaunchKernelGGL(k_adj_correct, dim3(blocks1d(nt)), dim3(256), 0, st, A.g.ny, A.g.nx, A.Lambda, C.Lambda);
+    TRY(check_launch("adjoint correction"));
+    TRY(compute_mask(A, st));
+    TRY(adjoint_sweeps(A, sc.adj_sweeps, 1.0, st));
+    return fill(A.mask, nc, 0.0, st);
+}
 }  // namespace
 
 // ===========================================================================
@@ -818,6 +1317,75 @@ int glide_local_activation(const GlideGeom *g, const double *r, const double *U,
     TRY(check_geom(g));
     if (!r || !U || !gamma || !active) return fail(GLIDE_E_NULL, "activation: NULL argument");
     return launch_activation(*g, r, U, gamma, active, S(stream));
+}
+
+
+int glide_jvp(const GlideGeom *g, const GlideParams *p, double *j, const double *U, const double *dU, const double *bed,
+              const double *B, const double *beta, const double *mask, const double *gamma, void *stream)
+{
+    TRY(check_geom(g));
+    if (!p || !j || !U || !dU || !bed || !B || !beta || !mask) return fail(GLIDE_E_NULL, "jvp: NULL argument");
+    (void)gamma;
+    hipLaunchKernelGGL(k_jvp, point_grid(g->ny, g->nx), dim3(TX, TY), 0, S(stream), *g, to_params(*p), j, U, dU, bed,
+                       B, beta, mask);
+    return check_launch("jvp");
+}
+
+int glide_vjp(const GlideGeom *g, const GlideParams *p, double *l, const double *U, const double *lam, const double *bed,
+              const double *B, const double *beta, const double *mask, const double *gamma, void *stream)
+{
+    TRY(check_geom(g));
+    if (!p || !l || !U || !lam || !bed || !B || !beta || !mask) return fail(GLIDE_E_NULL, "vjp: NULL argument");
+    (void)gamma;
+    return launch_vjp(*g, to_params(*p), l, nullptr, U, lam, bed, B, beta, mask, S(stream));
+}
+
+int glide_vanka_adjoint_pass(const GlideGeom *g, const GlideParams *p, double *lam_out, const double *lam,
+                             const double *mask, const double *r_adj, const double *U, const double *bed,
+                             const double *B, const double *beta, const double *gamma, int color, double omega,
+                             void *stream)
+{
+    TRY(check_geom(g));
+    if (!p || !lam_out || !lam || !mask || !r_adj || !U || !bed || !B || !beta || !gamma)
+        return fail(GLIDE_E_NULL, "vanka_adjoint_pass: NULL argument");
+    if (color != 0 && color != 1) return fail(GLIDE_E_ARG, "color must be 0 or 1");
+    hipLaunchKernelGGL(k_vanka_adj, cell_grid(g->ny, g->nx), dim3(TX, TY), 0, S(stream), *g, to_params(*p), lam_out,
+                       lam, mask, r_adj, U, bed, B, beta, gamma, color, omega);
+    return check_launch("vanka_adjoint_pass");
+}
+
+int glide_grad_beta(const GlideGeom *g, const GlideParams *p, double *grad, const double *U, const double *lam,
+                    const double *bed, const double *beta, int reference_edges, void *stream)
+{
+    TRY(check_geom(g));
+    if (!p || !grad || !U || !lam || !bed || !beta) return fail(GLIDE_E_NULL, "grad_beta: NULL argument");
+    hipLaunchKernelGGL(k_grad_beta, dim3(blocks1d(n_c(g->ny, g->nx))), dim3(256), 0, S(stream), *g, to_params(*p), grad,
+                       U, lam, bed, beta, reference_edges);
+    return check_launch("grad_beta");
+}
+
+int glide_adjoint_vcycle(const GlideLevel *L, int n, const GlideSchedule *sched, void *stream)
+{
+    TRY(check_adjoint_levels(L, n));
+    GlideSchedule sc = sched ? *sched : default_schedule();
+    return adjoint_vc(L, n, 0, sc, S(stream));
+}
+
+int glide_compute_mask(const GlideLevel *level, void *stream)
+{
+    if (!level) return fail(GLIDE_E_NULL, "compute_mask: NULL");
+    TRY(check_geom(&level->g));
+    if (!level->U || !level->mask || !level->gamma) return fail(GLIDE_E_NULL, "compute_mask: NULL buffer");
+    return compute_mask(*level, S(stream));
+}
+
+int glide_adjoint_residual(const GlideLevel *level, void *stream)
+{
+    if (!level) return fail(GLIDE_E_NULL, "adjoint_residual: NULL");
+    TRY(check_geom(&level->g));
+    const GlideLevel &A = *level;
+    if (!A.U || !A.Lambda || !A.f_adj || !A.r_adj || !A.mask) return fail(GLIDE_E_NULL, "adjoint_residual: NULL buffer");
+    return launch_vjp(A.g, to_params(A.p), A.r_adj, A.f_adj, A.U, A.Lambda, A.bed, A.B, A.beta, A.mask, S(stream));
 }
 
 int glide_restrict_u(const double *uf, double *uc, int nyc, int nxc, void *stream)

@@ -139,6 +139,50 @@ class IcePhysics:
             self.grid.H_prev.copy_(self.grid.H)
         return self.grid.u, self.grid.v, self.grid.H
 
+    # ------------------------------------------------------------ adjoint
+    def adjoint_vcycle(self):
+        sched = ctypes.byref(self.schedule) if self.schedule is not None else None
+        _lib.call("glide_adjoint_vcycle", self._levels(), self.n_levels, sched, self._stream())
+
+    def grad_beta(self, reference_edges: bool = True) -> torch.Tensor:
+        """lambda . dR/dbeta at the finest level (glide/grid.py:336-355).
+        reference_edges reproduces the reference's skipped edge-cell terms
+        (glide/cuda/grad.cu:52-53,70-71); False gives the exact transpose."""
+        g = self.grid
+        lev = self._levels()
+        grad = torch.empty((g.ny, g.nx), dtype=torch.float64, device=self.device)
+        _lib.call("glide_compute_mask", ctypes.byref(lev[0]), self._stream())
+        geom, prm = g.geom(), g.cparams()
+        _lib.call("glide_grad_beta", ctypes.byref(geom), ctypes.byref(prm), grad.data_ptr(), g.U.data_ptr(),
+                  g.Lambda.data_ptr(), g.bed.data_ptr(), g.beta.data_ptr(), int(bool(reference_edges)),
+                  self._stream())
+        g.mask.zero_()
+        return grad
+
+    def adjoint(self, dL_du, dL_dv, dL_dH, n_vcycles=2, reference_edges=True):
+        """Gradient of a loss w.r.t. beta by the adjoint V-cycles
+        (glide/physics.py:211-242): f_adj = -dL/dU, Lambda = 0, n adjoint
+        V-cycles about the current (converged) state, then lambda . dR/dbeta."""
+        g = self.grid
+        fu, fv, fH = g.fields(g.f_adj)
+        fu.copy_(-_as_f64(dL_du, self.device).reshape(fu.shape))
+        fv.copy_(-_as_f64(dL_dv, self.device).reshape(fv.shape))
+        fH.copy_(-_as_f64(dL_dH, self.device).reshape(fH.shape))
+        g.Lambda.zero_()
+        g.Lambda_out.zero_()
+        for _ in range(n_vcycles):
+            self.adjoint_vcycle()
+        return self.grad_beta(reference_edges)
+
+    def adjoint_residual_norm(self) -> float:
+        """||f_adj - J^T Lambda|| at the finest level with the adjoint's mask."""
+        g = self.grid
+        lev = self._levels()
+        _lib.call("glide_compute_mask", ctypes.byref(lev[0]), self._stream())
+        _lib.call("glide_adjoint_residual", ctypes.byref(lev[0]), self._stream())
+        g.mask.zero_()
+        return float(torch.linalg.norm(g.r_adj))
+
     # ------------------------------------------------------------ misc API
     def reset_solution(self):
         """glide/physics.py:244-248"""

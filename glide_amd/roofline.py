@@ -1,0 +1,29 @@
+"""Algorithmic work per unit of the solver's kernels (used by bench.py).
+
+FP64 flop counts are counted from the kernels' own source by
+tests/test_flop_count.py (every +, -, *, / is one flop, fma two, min/max/abs/
+copysign/floor none; exp and log are counted by the operations of their
+implementation in detmath.h), for an interior cell, unconstrained mass row,
+gl_derivatives off.
+"""
+
+# one local Newton iteration of one cell: 5x5 block assembly + LU solve + update
+FLOPS_NEWTON_ITER = 1043
+# viscosity of one cell (computed once per sweep)
+FLOPS_ETA = 122
+NEWTON_ITERS = 10
+
+PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
+
+
+def sweep_flops(n_cells: int, n_newton: int = NEWTON_ITERS) -> int:
+    return n_cells * (FLOPS_ETA + n_newton * FLOPS_NEWTON_ITER)
+
+
+def residual_bytes(ny: int, nx: int) -> int:
+    """Compulsory HBM bytes of one residual evaluation: read U and f (state
+    vectors), bed, B, beta, mask, gamma (cells); write r (state vector)."""
+    nu, nv, nc = ny * (nx + 1), (ny + 1) * nx, ny * nx
+    nt = nu + nv + nc
+    return 8 * (3 * nt + 5 * nc)

@@ -185,6 +185,14 @@ int glide_residual_norm(const GlideLevel *level, double *out_host, void *stream)
 int glide_adjoint_vcycle(const GlideLevel *levels, int n_levels, const GlideSchedule *sched,
                          void *stream);
 
+/* mask = (H <= gamma + 0.1) on one level: the adjoint's active set
+ * (glide/grid.py:332-334). */
+int glide_compute_mask(const GlideLevel *level, void *stream);
+
+/* r_adj = f_adj - J(U)^T Lambda on one level, with the level's current mask
+ * (glide/solver.py:187-189, glide/grid.py:322-324). */
+int glide_adjoint_residual(const GlideLevel *level, void *stream);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *glide_last_error(void);
 

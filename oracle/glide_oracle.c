@@ -931,6 +931,11 @@ typedef struct {
     const double *u, *v, *H, *f_u, *f_v, *f_H, *bed, *beta, *gamma, *eta;
 } VCtx;
 
+/* Experimental variant (not the reference algorithm): add the shear
+ * cross-couplings that the reference's block leaves out. */
+static int g_full_block = 0;
+void or_set_full_block(int on) { g_full_block = on; }
+
 static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, double v_t,
                            double v_b, double H_c, double thklim, int mode, int adj_masked,
                            double J[25], double r[5])
@@ -1048,6 +1053,7 @@ static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, 
         r[2] -= s.res * dx_inv;
         J[12] -= s.d_v_r * dx_inv;
         J[14] -= s.d_eta_H * dH_br * dx_inv;
+        if (g_full_block) { J[2] += s.d_v_r * dx_inv; J[10] -= s.d_u_b * dx_inv; }
     }
     /* top-right vertex (i, j+1) */
     {
@@ -1063,6 +1069,7 @@ static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, 
         r[2] += s.res * dx_inv;
         J[12] += s.d_v_l * dx_inv;
         J[14] += s.d_eta_H * dH_bl * dx_inv;
+        if (g_full_block) { J[7] += s.d_v_l * dx_inv; J[11] += s.d_u_b * dx_inv; }
     }
     /* bottom-left vertex (i+1, j) */
     {
@@ -1078,6 +1085,7 @@ static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, 
         r[3] -= s.res * dx_inv;
         J[18] -= s.d_v_r * dx_inv;
         J[19] -= s.d_eta_H * dH_tr * dx_inv;
+        if (g_full_block) { J[3] -= s.d_v_r * dx_inv; J[15] -= s.d_u_t * dx_inv; }
     }
     /* bottom-right vertex (i+1, j+1) */
     {
@@ -1093,6 +1101,7 @@ static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, 
         r[3] += s.res * dx_inv;
         J[18] += s.d_v_l * dx_inv;
         J[19] += s.d_eta_H * dH_tl * dx_inv;
+        if (g_full_block) { J[8] -= s.d_v_l * dx_inv; J[16] += s.d_u_t * dx_inv; }
     }
     /* basal drag */
     {

@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports every entry point include/glide_hip.h
+declares, and the ctypes table binds exactly those (no GPU work here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from glide_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "glide_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(glide_[A-Za-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_table_matches_header():
+    assert sorted(_lib.SIGNATURES) == declared()
+
+
+def test_errors_are_reported_without_a_gpu():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    # argument validation runs before any device work
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_residual", None, None, None, None, None, None, None, None, None, None, None, None)
+    assert e.value.code == _lib.GLIDE_E_NULL
+    g = _lib.GlideGeom(0, 4, 1.0, 1.0)
+    p = _lib.GlideParams(3.0, 1e-5, 1e-3, 1.0, 0.1, 0)
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_residual", ctypes.byref(g), ctypes.byref(p), 1, 1, None, None, 1, 1, 1, 1, 1, None)
+    assert e.value.code == _lib.GLIDE_E_SIZE
+    g = _lib.GlideGeom(8, 8, -1.0, 1.0)
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_residual", ctypes.byref(g), ctypes.byref(p), 1, 1, None, None, 1, 1, 1, 1, 1, None)
+    assert e.value.code == _lib.GLIDE_E_ARG
+    assert _lib.load().glide_version().startswith(b"glide_amd")
