@@ -276,6 +276,7 @@ __global__ __launch_bounds__(NT) void k_vanka(GlideGeom g, Params p, const doubl
     const double dx_inv = 1.0 / g.dx, dt = g.dt;
 
     CellNbhd N = load_nbhd(i, j, ny, nx, nu, nv, u, v, H, f, bed, beta, E);
+    nbhd_grounding(N, p);
     const double u_l0 = ld_u(u, i, j, ny, nx), u_r0 = ld_u(u, i, j + 1, ny, nx);
     const double v_t0 = ld_v(v, i, j, ny, nx), v_b0 = ld_v(v, i + 1, j, ny, nx);
     const double H_c0 = ld_c(H, i, j, ny, nx), th = ld_c(gamma, i, j, ny, nx);
@@ -320,6 +321,7 @@ __global__ __launch_bounds__(NT) void k_vanka_block(GlideGeom g, Params p, const
     auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
     const double dx_inv = 1.0 / g.dx, dt = g.dt;
     CellNbhd N = load_nbhd(i, j, ny, nx, nu, nv, u, v, H, f, bed, beta, E);
+    nbhd_grounding(N, p);
     double u_l = ld_u(u, i, j, ny, nx), u_r = ld_u(u, i, j + 1, ny, nx);
     double v_t = ld_v(v, i, j, ny, nx), v_b = ld_v(v, i + 1, j, ny, nx);
     double H_c = ld_c(H, i, j, ny, nx);
@@ -953,6 +955,7 @@ __global__ __launch_bounds__(NT) void k_vanka_adj(GlideGeom g, Params p, double 
     auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
     const double dx_inv = 1.0 / g.dx, dt = g.dt;
     CellNbhd N = load_nbhd(i, j, ny, nx, nu, nv, u, v, H, (const double *)nullptr, bed, beta, E);
+    nbhd_grounding(N, p);
     double J[25], r[5], JT[25], rhs[5], d[5];
     const bool masked = ld_c(mask, i, j, ny, nx) > 0.5;
     vanka_assemble<1>(N, p, i, j, ny, nx, dx_inv, dt, ld_u(u, i, j, ny, nx), ld_u(u, i, j + 1, ny, nx),

@@ -182,15 +182,13 @@ __host__ __device__ __forceinline__ SigS sigma_xy(double u_t, double u_b, double
     J.d_eta_H = 2.0 * eps_xy;
     return J;
 }
-// Basal drag on a face between cells l and r (or t and b) (stress.cu:245-361)
-__host__ __device__ __forceinline__ TauB tau_b(double vel, double H_l, double H_r, double bed_l, double bed_r,
-                                      double beta_l, double beta_r, const Params &p)
+// Basal drag on a face between cells l and r (or t and b), given the two
+// cells' grounding sigmoids g = sigm(bed + 0.917 H) and, with gl_derivatives,
+// their derivatives sigm_d (stress.cu:245-361).
+__host__ __device__ __forceinline__ TauB tau_b_g(double vel, double g_l, double g_r, double gd_l, double gd_r,
+                                                 double beta_l, double beta_r, const Params &p)
 {
     TauB J = {0, 0, 0, 0, 0, 0};
-    double z_l = bed_l + 0.917 * H_l;
-    double z_r = bed_r + 0.917 * H_r;
-    double g_l = sigm(z_l, p.gl_c);
-    double g_r = sigm(z_r, p.gl_c);
     double be_l = g_l * beta_l + (1.0 - g_l) * p.water_drag;
     double be_r = g_r * beta_r + (1.0 - g_r) * p.water_drag;
     double be = 0.5 * (be_l + be_r);
@@ -199,14 +197,30 @@ __host__ __device__ __forceinline__ TauB tau_b(double vel, double H_l, double H_
     J.d_beta_l = -0.5 * g_l * vel;
     J.d_beta_r = -0.5 * g_r * vel;
     if (p.gl_derivatives) {
-        double dg_l = 0.917 * sigm_d(z_l, p.gl_c);
-        double dg_r = 0.917 * sigm_d(z_r, p.gl_c);
+        double dg_l = 0.917 * gd_l;
+        double dg_r = 0.917 * gd_r;
         double dbe_l = dg_l * (beta_l - p.water_drag);
         double dbe_r = dg_r * (beta_r - p.water_drag);
         J.d_H_l = -0.5 * dbe_l * vel;
         J.d_H_r = -0.5 * dbe_r * vel;
     }
     return J;
+}
+// grounding sigmoid of one cell and its derivative (0 without gl_derivatives)
+struct Ground { double g, gd; };
+__host__ __device__ __forceinline__ Ground ground(double bed, double H, const Params &p)
+{
+    double z = bed + 0.917 * H;
+    Ground o;
+    o.g = sigm(z, p.gl_c);
+    o.gd = p.gl_derivatives ? sigm_d(z, p.gl_c) : 0.0;
+    return o;
+}
+__host__ __device__ __forceinline__ TauB tau_b(double vel, double H_l, double H_r, double bed_l, double bed_r,
+                                               double beta_l, double beta_r, const Params &p)
+{
+    Ground a = ground(bed_l, H_l, p), b = ground(bed_r, H_r, p);
+    return tau_b_g(vel, a.g, b.g, a.gd, b.gd, beta_l, beta_r, p);
 }
 // Driving stress on u face column j (stress.cu:407-436)
 __host__ __device__ __forceinline__ TauD tau_dx(double H_l, double H_r, double bed_l, double bed_r, double dx_inv,
@@ -273,19 +287,21 @@ __host__ __device__ __forceinline__ Flux flux_y(double v, double H_t, double H_b
     J.d_u = H_avg - 0.5 * vs * (H_t - H_b);
     return J;
 }
-// Calving sink of floating ice (flux.cu:188-211)
-__host__ __device__ __forceinline__ Calv calving(double H, double bed, const Params &p)
+// Calving sink of floating ice (flux.cu:188-211), given the cell's sigmoid
+__host__ __device__ __forceinline__ Calv calving_g(double H, const Ground &gr, const Params &p)
 {
     Calv J;
-    double z = bed + 0.917 * H;
-    double g = sigm(z, p.gl_c);
-    J.res = -p.calving_rate * (1.0 - g) * H;
-    J.d_H = -p.calving_rate * (1.0 - g);
+    J.res = -p.calving_rate * (1.0 - gr.g) * H;
+    J.d_H = -p.calving_rate * (1.0 - gr.g);
     if (p.gl_derivatives) {
-        double dg = 0.917 * sigm_d(z, p.gl_c);
+        double dg = 0.917 * gr.gd;
         J.d_H += p.calving_rate * H * dg;
     }
     return J;
+}
+__host__ __device__ __forceinline__ Calv calving(double H, double bed, const Params &p)
+{
+    return calving_g(H, ground(bed, H, p), p);
 }
 
 __host__ __device__ __forceinline__ double sigN_jvp(const SigN &J, double ul, double ur, double vt, double vb, double eH)

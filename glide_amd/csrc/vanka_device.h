@@ -56,7 +56,17 @@ struct CellNbhd {
     double u_ll, u_rr, u_tl, u_tr, u_bl, u_br;   // u[i,j-1], u[i,j+2], u[i-1,j], u[i-1,j+1], u[i+1,j], u[i+1,j+1]
     double v_lt, v_lb, v_rt, v_rb, v_tt, v_bb;   // v[i,j-1], v[i+1,j-1], v[i,j+1], v[i+1,j+1], v[i-1,j], v[i+2,j]
     double f0, f1, f2, f3, f4;                   // right-hand side at the 5 dofs
+    // grounding sigmoids of the 4 edge neighbours (fixed while the cell iterates)
+    Ground g_l, g_r, g_t, g_b;
 };
+
+__host__ __device__ __forceinline__ void nbhd_grounding(CellNbhd &N, const Params &p)
+{
+    N.g_l = ground(N.bed_l, N.H_l, p);
+    N.g_r = ground(N.bed_r, N.H_r, p);
+    N.g_t = ground(N.bed_t, N.H_t, p);
+    N.g_b = ground(N.bed_b, N.H_b, p);
+}
 
 // MODE 0: forward Vanka (residual, active-set test, diagonal shift)
 // MODE 1: adjoint block (no residual, mass row replaced where adj_masked)
@@ -70,6 +80,7 @@ __host__ __device__ __forceinline__ void vanka_assemble(const CellNbhd &N, const
     for (int k = 0; k < 25; k++) J[k] = 0.0;
 #pragma unroll
     for (int k = 0; k < 5; k++) r[k] = 0.0;
+    const Ground g_c = ground(N.bed_c, H_c, p);   // the only sigmoid that depends on the iterate
     if (MODE == 0) {
         r[0] -= N.f0;
         r[1] -= N.f1;
@@ -80,7 +91,7 @@ __host__ __device__ __forceinline__ void vanka_assemble(const CellNbhd &N, const
     {
         J[24] = 1.0 / dt;
         r[4] += H_c / dt;
-        Calv jc = calving(H_c, N.bed_c, p);
+        Calv jc = calving_g(H_c, g_c, p);
         J[24] -= jc.d_H;
         r[4] -= jc.res;
         Flux q = flux_x(u_l, N.H_l, H_c, j, nx);
@@ -189,19 +200,19 @@ __host__ __device__ __forceinline__ void vanka_assemble(const CellNbhd &N, const
         J[19] += s.d_eta_H * dH * dx_inv;
     }
     {
-        TauB t = tau_b(u_l, N.H_l, H_c, N.bed_l, N.bed_c, N.beta_l, N.beta_c, p);
+        TauB t = tau_b_g(u_l, N.g_l.g, g_c.g, N.g_l.gd, g_c.gd, N.beta_l, N.beta_c, p);
         r[0] += t.res; J[0] += t.d_u; J[4] += t.d_H_r;
     }
     {
-        TauB t = tau_b(u_r, H_c, N.H_r, N.bed_c, N.bed_r, N.beta_c, N.beta_r, p);
+        TauB t = tau_b_g(u_r, g_c.g, N.g_r.g, g_c.gd, N.g_r.gd, N.beta_c, N.beta_r, p);
         r[1] += t.res; J[6] += t.d_u; J[9] += t.d_H_l;
     }
     {
-        TauB t = tau_b(v_t, N.H_t, H_c, N.bed_t, N.bed_c, N.beta_t, N.beta_c, p);
+        TauB t = tau_b_g(v_t, N.g_t.g, g_c.g, N.g_t.gd, g_c.gd, N.beta_t, N.beta_c, p);
         r[2] += t.res; J[12] += t.d_u; J[14] += t.d_H_r;
     }
     {
-        TauB t = tau_b(v_b, H_c, N.H_b, N.bed_c, N.bed_b, N.beta_c, N.beta_b, p);
+        TauB t = tau_b_g(v_b, g_c.g, N.g_b.g, g_c.gd, N.g_b.gd, N.beta_c, N.beta_b, p);
         r[3] += t.res; J[18] += t.d_u; J[19] += t.d_H_l;
     }
     {

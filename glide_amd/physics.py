@@ -174,6 +174,39 @@ class IcePhysics:
             self.adjoint_vcycle()
         return self.grad_beta(reference_edges)
 
+    def adjoint_solve(self, dL_du, dL_dv, dL_dH, tol=1e-12, max_vcycles=200):
+        """Adjoint V-cycles until ||f_adj - J^T Lambda|| <= tol * ||f_adj||
+        (or max_vcycles); the relative residual history is kept in
+        last_adj_history."""
+        g = self.grid
+        fu, fv, fH = g.fields(g.f_adj)
+        fu.copy_(-_as_f64(dL_du, self.device).reshape(fu.shape))
+        fv.copy_(-_as_f64(dL_dv, self.device).reshape(fv.shape))
+        fH.copy_(-_as_f64(dL_dH, self.device).reshape(fH.shape))
+        g.Lambda.zero_()
+        g.Lambda_out.zero_()
+        r0 = float(torch.linalg.norm(g.f_adj))
+        self.last_adj_history = []
+        if r0 == 0.0:
+            return
+        for _ in range(max_vcycles):
+            self.adjoint_vcycle()
+            rel = self.adjoint_residual_norm() / r0
+            self.last_adj_history.append(rel)
+            if rel <= tol:
+                break
+
+    def grad_H_prev(self) -> torch.Tensor:
+        """dL/dH_prev = -Lambda_H / dt outside the adjoint's active set
+        (f_H = H_prev / dt + smb, glide/physics.py:173)."""
+        g = self.grid
+        lev = self._levels()
+        _lib.call("glide_compute_mask", ctypes.byref(lev[0]), self._stream())
+        _, _, lH = g.fields(g.Lambda)
+        out = -lH / g.dt * (1.0 - g.mask)
+        g.mask.zero_()
+        return out
+
     def adjoint_residual_norm(self) -> float:
         """||f_adj - J^T Lambda|| at the finest level with the adjoint's mask."""
         g = self.grid

@@ -7,10 +7,13 @@ implementation in detmath.h), for an interior cell, unconstrained mass row,
 gl_derivatives off.
 """
 
-# one local Newton iteration of one cell: 5x5 block assembly + LU solve + update
-FLOPS_NEWTON_ITER = 1043
+# one local Newton iteration of one cell: 5x5 block assembly (including the
+# centre cell's grounding sigmoid) + LU solve + update
+FLOPS_NEWTON_ITER = 739
 # viscosity of one cell (computed once per sweep)
 FLOPS_ETA = 122
+# per cell and sweep: grounding sigmoids of the 4 edge neighbours
+FLOPS_CELL_SETUP = 152
 NEWTON_ITERS = 10
 
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
@@ -18,7 +21,7 @@ PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
 
 
 def sweep_flops(n_cells: int, n_newton: int = NEWTON_ITERS) -> int:
-    return n_cells * (FLOPS_ETA + n_newton * FLOPS_NEWTON_ITER)
+    return n_cells * (FLOPS_ETA + FLOPS_CELL_SETUP + n_newton * FLOPS_NEWTON_ITER)
 
 
 def residual_bytes(ny: int, nx: int) -> int:

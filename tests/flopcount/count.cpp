@@ -70,6 +70,9 @@ int main()
     N.H_l = N.H_r = N.H_t = N.H_b = N.H_tl = N.H_tr = N.H_bl = N.H_br = R(700.0);
     N.bed_c = N.bed_l = N.bed_r = N.bed_t = N.bed_b = R(300.0);
     N.beta_c = N.beta_l = N.beta_r = N.beta_t = N.beta_b = R(0.1);
+    g_flops = 0;
+    nbhd_grounding(N, p);
+    long setup_flops = g_flops;
     R J[25], r[5], d[5];
     R ul(40.0), ur(60.0), vt(-20.0), vb(10.0), Hc(700.0), th(0.1);
     g_flops = 0;
@@ -83,6 +86,6 @@ int main()
     Hc -= 0.25 * d[4];
     Hc = fmax(Hc, th);
     bool constrained = (Hc - R(10.0) * r[4]) <= th;
-    printf("%ld %ld %d\n", g_flops, eta_flops, (int)constrained);
+    printf("%ld %ld %d %ld\n", g_flops, eta_flops, (int)constrained, setup_flops);
     return 0;
 }

@@ -43,15 +43,20 @@ def counts(tmp_path_factory):
     subprocess.run(["g++", "-std=c++17", "-O0", "-I", str(d), "-I", os.path.join(ROOT, "include"),
                     "-o", str(exe), os.path.join(HERE, "flopcount", "count.cpp")], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
-    return int(out[0]), int(out[1]), int(out[2])
+    return int(out[0]), int(out[1]), int(out[2]), int(out[3])
 
 
 def test_newton_iteration_flops(counts):
-    newton, _, constrained = counts
+    newton, _, constrained, _ = counts
     assert constrained == 0
     assert newton == roofline.FLOPS_NEWTON_ITER, newton
 
 
 def test_eta_flops(counts):
-    _, eta, _ = counts
+    _, eta, _, _ = counts
     assert eta == roofline.FLOPS_ETA, eta
+
+
+def test_cell_setup_flops(counts):
+    _, _, _, setup = counts
+    assert setup == roofline.FLOPS_CELL_SETUP, setup

@@ -335,3 +335,36 @@ def test_error_mask_and_local_activation():
     em2[13, 14] = 1.0        # on the halo ring of blocks (0,0),(0,1),(1,0),(1,1)
     act2 = O.local_activation(em2)
     assert act2[:28, :28].all() and act2.sum() == 28 * 28
+
+
+def _fd4(fun, U, d, e):
+    return (8 * (fun(U + e * d) - fun(U - e * d)) - (fun(U + 2 * e * d) - fun(U - 2 * e * d))) / (12 * e)
+
+
+def test_jvp_fourth_order_fd_on_saturated_state():
+    """On a grounded state (sigmoid clamped flat everywhere) the JVP without
+    grounding-line derivatives is the exact derivative: 4th-order central FD
+    agrees to ~1e-10.  With gl_derivatives the reference's sigmoid derivative
+    ignores the +-20 clamp (common.cu:67-76), so saturated cells carry a spurious
+    c*H*0.917*s(1-s) in the calving Jacobian (flux.cu:205-208): the H columns
+    are then off at the 1e-7 level, which this test also pins."""
+    n = 24
+    from glide_amd import synthetic as S
+    w = S.ice_slab(n, n, 2000.0, seed=2)
+    rng = np.random.default_rng(0)
+    U = O.join(rng.normal(0, 50, (n, n + 1)), rng.normal(0, 50, (n + 1, n)), w["H"])
+    f = np.zeros_like(U)
+    mask, gamma = np.zeros((n, n)), np.full((n, n), 0.1)
+    d = rng.normal(size=U.size)
+    d[-n * n:] *= 10.0
+    errs = {}
+    for gl in (False, True):
+        prm = O.params(gl_derivatives=gl)
+
+        def R(X):
+            return O.residual(n, n, w["dx"], 10.0, prm, X, f, w["bed"], w["B"], w["beta"], mask, gamma)
+        jv = O.jvp(n, n, w["dx"], 10.0, prm, U, d, w["bed"], w["B"], w["beta"], mask, gamma)
+        fd = _fd4(R, U, d, 1e-3)
+        errs[gl] = np.linalg.norm(jv - fd) / np.linalg.norm(fd)
+    assert errs[False] <= 1e-9, errs
+    assert 1e-9 < errs[True] <= 1e-5, errs
