@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--workload", default="cap_512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-member batched ensemble")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the kernel microbenchmarks (for profiling them alone)")
     return ap.parse_args()
@@ -119,6 +120,79 @@ def cpu_baseline(w, threads: int):
             "sample": f"1 V-cycle of step 2 of {w['name']} timed ({dt_cycle:.1f} s), x{w['n_vcycles']} V-cycles per step"}
 
 
+def ensemble_bench(dt=10.0, n_vcycles=3):
+    """The batched ensemble: 64 ice caps (16 each at 256^2, 384x512, 512^2,
+    768^2; seeds 1000-1063; dx 1500 m), one FASCD step of 3 V-cycles for all of
+    them in batched persistent launches, against the same step run member by
+    member; plus the ensemble-wide residual stencil sweep against HBM peak."""
+    import numpy as np
+    import torch
+
+    from glide_amd import synthetic as S
+    from glide_amd.autotune import tune
+    from glide_amd.ensemble import BK_RESIDUAL, Ensemble
+    from glide_amd.physics import IcePhysics
+
+    members = S.ensemble_members()
+    phs = []
+    for d in members:
+        ph = IcePhysics(d["ny"], d["nx"], d["dx"], n_levels=d["n_levels"])
+        ph.set_geometry(d["bed"], d["H"])
+        ph.set_parameters(B=d["B"], beta=d["beta"], smb=d["smb"])
+        phs.append(ph)
+    ens = Ensemble(phs)
+    t0 = time.perf_counter()
+    rec = tune(ens, reps=3, use_cache=False)
+    tune_s = time.perf_counter() - t0
+    ens.forward(dt, n_vcycles=n_vcycles)          # warm-up step
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.forward(dt, n_vcycles=n_vcycles)
+    torch.cuda.synchronize()
+    batched_ms = (time.perf_counter() - t0) * 1e3
+    solo_ms = []
+    for ph in phs:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ph.forward(dt, n_vcycles=n_vcycles)
+        torch.cuda.synchronize()
+        solo_ms.append((time.perf_counter() - t0) * 1e3)
+    # ensemble-wide residual on every finest level (one launch)
+    fin = ens.n_phases
+    ty = int(rec["choice"]["residual"][str(fin)])
+    res_ms = ens.time_kernel(BK_RESIDUAL, fin, ty, 0, reps=20)
+    res_bytes = sum(residual_bytes(d["ny"], d["nx"]) for d in members)
+    res_gbs = res_bytes / (res_ms * 1e-3) / 1e9
+    cells = sum(d["ny"] * d["nx"] for d in members)
+    solo = np.array(solo_ms)
+    by_size = {}
+    for d, t in zip(members, solo_ms):
+        by_size.setdefault(f"{d['ny']}x{d['nx']}", []).append(t)
+    out = {
+        "members": len(members), "finest_cells": cells, "vcycles_per_step": n_vcycles,
+        "levels": sorted({d["n_levels"] for d in members}), "phases": ens.n_phases,
+        "batched_ms_per_step": round(batched_ms, 1),
+        "batched_member_latency_ms": {"min": round(batched_ms, 1), "max": round(batched_ms, 1),
+                                      "note": "every member completes with the batched step"},
+        "sequential_ms_per_step": round(float(solo.sum()), 1),
+        "sequential_member_latency_ms": {"min": round(float(solo.min()), 1), "median": round(float(np.median(solo)), 1),
+                                         "max": round(float(solo.max()), 1),
+                                         "by_size_mean": {k: round(float(np.mean(v)), 1) for k, v in by_size.items()}},
+        "speedup_batched_vs_sequential": round(float(solo.sum()) / batched_ms, 3),
+        "throughput_member_steps_per_s": round(len(members) / (batched_ms * 1e-3), 2),
+        "throughput_cell_vcycles_per_s": float(f"{cells * n_vcycles / (batched_ms * 1e-3):.4e}"),
+        "autotune_s": round(tune_s, 1), "tile_heights": rec["choice"],
+        "stencil_roofline": {"kernel": "kb_residual (every finest level, one launch)", "bound": "hbm",
+                             "achieved": round(res_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": round(res_gbs / PEAK_HBM_GBS, 4), "ms_per_launch": round(res_ms, 4),
+                             "work": f"{res_bytes} algorithmic bytes", "tile_height": ty},
+    }
+    ens.close()
+    del phs
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -163,6 +237,10 @@ def main():
         "reaches_1e-8": bool(hist and min(hist) <= 1e-8),
         **rl,
     }
+    if not args.no_ensemble:
+        del ph
+        torch.cuda.empty_cache()
+        line["ensemble"] = ensemble_bench()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, args.cpu_threads)
     print(json.dumps(line))

@@ -76,6 +76,16 @@ SIGNATURES = {
     "glide_adjoint_vcycle": (c_int, [POINTER(GlideLevel), c_int, POINTER(GlideSchedule), c_void_p]),
     "glide_compute_mask": (c_int, [POINTER(GlideLevel), c_void_p]),
     "glide_adjoint_residual": (c_int, [POINTER(GlideLevel), c_void_p]),
+    "glide_batch_create": (c_int, [POINTER(POINTER(GlideLevel)), POINTER(c_int), c_int, POINTER(c_void_p)]),
+    "glide_batch_destroy": (c_int, [c_void_p]),
+    "glide_batch_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+    "glide_batch_forward_setup": (c_int, [c_void_p, c_double, c_void_p]),
+    "glide_batch_vcycle": (c_int, [c_void_p, c_double, POINTER(GlideSchedule), c_void_p]),
+    "glide_batch_residual_norms": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
+    "glide_batch_residual": (c_int, [c_void_p, c_int, c_void_p]),
+    "glide_batch_set_enabled": (c_int, [c_void_p, POINTER(ctypes.c_ubyte), c_void_p]),
+    "glide_batch_set_tiles": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "glide_batch_time_kernel": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, POINTER(c_double), c_void_p]),
     "glide_last_error": (c_char_p, []),
     "glide_version": (c_char_p, []),
 }

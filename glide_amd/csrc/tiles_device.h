@@ -1,0 +1,363 @@
+// tiles_device.h - tile bodies shared by the single-level and the batched
+// (ensemble) kernels: the cell-viscosity LDS tile, the residual stencil, the
+// forward Vanka smoother, its gather-apply pass and the local-smoother block
+// activation.  Each body works on one (bx, by) tile of TX x TY points or
+// cells with a (TX, TY) thread block, so a kernel may run it once per
+// workgroup (one tile per block) or in a loop over many tiles (persistent
+// batched launches).  The per-point arithmetic does not depend on the tile
+// shape, so every shape gives bitwise-identical results.
+#pragma once
+
+#include "vanka_device.h"
+
+namespace glide {
+
+// records msg as glide_last_error() and returns code (host side)
+int set_error(int code, const char *msg);
+
+constexpr int RB = 14;  // reference launch-block interior (glide/grid.py:184-186)
+
+__host__ __device__ inline size_t n_u(int ny, int nx) { return (size_t)ny * (nx + 1); }
+__host__ __device__ inline size_t n_v(int ny, int nx) { return (size_t)(ny + 1) * nx; }
+__host__ __device__ inline size_t n_c(int ny, int nx) { return (size_t)ny * nx; }
+__host__ __device__ inline size_t n_t(int ny, int nx) { return n_u(ny, nx) + n_v(ny, nx) + n_c(ny, nx); }
+
+// ---------------------------------------------------------------------------
+// viscosity tile: cells [i0-1, i0+TY] x [j0-1, j0+TX] into LDS (0 outside)
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+__device__ __forceinline__ void fill_eta_tile(double (*eta_s)[TX + 2], int i0, int j0, const double *__restrict__ u,
+                                              const double *__restrict__ v, const double *__restrict__ B,
+                                              const Params &p, double dx, int ny, int nx)
+{
+    int tid = threadIdx.y * TX + threadIdx.x;
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += TX * TY) {
+        int a = k / (TX + 2), b = k - a * (TX + 2);
+        int ci = i0 - 1 + a, cj = j0 - 1 + b;
+        eta_s[a][b] = (ci >= 0 && ci < ny && cj >= 0 && cj < nx) ? eta_cell(ci, cj, u, v, B, p, dx, ny, nx) : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Residual r = F(U) - f (- sub)  (glide/cuda/residuals.cu:6-292)
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+__device__ __forceinline__ void residual_tile(const GlideGeom &g, const Params &p, double *__restrict__ r,
+                                              const double *__restrict__ U, const double *__restrict__ f,
+                                              const double *__restrict__ sub, const double *__restrict__ bed,
+                                              const double *__restrict__ B, const double *__restrict__ beta,
+                                              const double *__restrict__ mask, const double *__restrict__ gamma,
+                                              int bx, int by, double (*eta_s)[TX + 2])
+{
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv;
+    const int i0 = by * TY, j0 = bx * TX;
+    fill_eta_tile<TX, TY>(eta_s, i0, j0, u, v, B, p, g.dx, ny, nx);
+    __syncthreads();
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    if (i > ny || j > nx) return;
+    auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+
+    if (i < ny && j < nx) {
+        double H_c = ld_c(H, i, j, ny, nx);
+        double rH = H_c / dt - (f ? ld_c(f + nu + nv, i, j, ny, nx) : 0.0);
+        Calv jc = calving(H_c, ld_c(bed, i, j, ny, nx), p);
+        rH -= jc.res;
+        double H_l = ld_c(H, i, j - 1, ny, nx);
+        Flux q = flux_x(ld_u(u, i, j, ny, nx), H_l, H_c, j, nx);
+        rH -= q.res * dx_inv;
+        double H_r = ld_c(H, i, j + 1, ny, nx);
+        q = flux_x(ld_u(u, i, j + 1, ny, nx), H_c, H_r, j + 1, nx);
+        rH += q.res * dx_inv;
+        double H_t = ld_c(H, i - 1, j, ny, nx);
+        q = flux_y(ld_v(v, i, j, ny, nx), H_t, H_c, i, ny);
+        rH += q.res * dx_inv;
+        double H_b = ld_c(H, i + 1, j, ny, nx);
+        q = flux_y(ld_v(v, i + 1, j, ny, nx), H_c, H_b, i + 1, ny);
+        rH -= q.res * dx_inv;
+        double m = ld_c(mask, i, j, ny, nx);
+        double th = ld_c(gamma, i, j, ny, nx);
+        double val = (1.0 - m) * rH + m * (H_c - th);
+        size_t k = nu + nv + (size_t)i * nx + j;
+        r[k] = sub ? val - sub[k] : val;
+    }
+    if (i < ny) {
+        double ru = f ? -ld_u(f, i, j, ny, nx) : -0.0;
+        {
+            double eH = E(i, j) * ld_c(H, i, j, ny, nx);
+            SigN s = sigma_xx(ld_u(u, i, j, ny, nx), ld_u(u, i, j + 1, ny, nx), ld_v(v, i, j, ny, nx),
+                              ld_v(v, i + 1, j, ny, nx), eH, dx_inv, j, nx);
+            ru += s.res * dx_inv;
+        }
+        {
+            double eH = E(i, j - 1) * ld_c(H, i, j - 1, ny, nx);
+            SigN s = sigma_xx(ld_u(u, i, j - 1, ny, nx), ld_u(u, i, j, ny, nx), ld_v(v, i, j - 1, ny, nx),
+                              ld_v(v, i + 1, j - 1, ny, nx), eH, dx_inv, j - 1, nx);
+            ru -= s.res * dx_inv;
+        }
+        {
+            double eH = etaH_vertex(E(i - 1, j - 1), E(i - 1, j), E(i, j - 1), E(i, j), ld_c(H, i - 1, j - 1, ny, nx),
+                                    ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx));
+            SigS s = sigma_xy(ld_u(u, i - 1, j, ny, nx), ld_u(u, i, j, ny, nx), ld_v(v, i, j - 1, ny, nx),
+                              ld_v(v, i, j, ny, nx), eH, dx_inv, i, j, ny, nx);
+            ru += s.res * dx_inv;
+        }
+        {
+            double eH = etaH_vertex(E(i, j - 1), E(i, j), E(i + 1, j - 1), E(i + 1, j), ld_c(H, i, j - 1, ny, nx),
+                                    ld_c(H, i, j, ny, nx), ld_c(H, i + 1, j - 1, ny, nx), ld_c(H, i + 1, j, ny, nx));
+            SigS s = sigma_xy(ld_u(u, i, j, ny, nx), ld_u(u, i + 1, j, ny, nx), ld_v(v, i + 1, j - 1, ny, nx),
+                              ld_v(v, i + 1, j, ny, nx), eH, dx_inv, i + 1, j, ny, nx);
+            ru -= s.res * dx_inv;
+        }
+        {
+            TauB t = tau_b(ld_u(u, i, j, ny, nx), ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx),
+                           ld_c(bed, i, j - 1, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i, j - 1, ny, nx),
+                           ld_c(beta, i, j, ny, nx), p);
+            ru += t.res;
+        }
+        {
+            TauD t = tau_dx(ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx), ld_c(bed, i, j - 1, ny, nx),
+                            ld_c(bed, i, j, ny, nx), dx_inv, j, nx);
+            ru -= t.res;
+        }
+        size_t k = (size_t)i * (nx + 1) + j;
+        r[k] = sub ? ru - sub[k] : ru;
+    }
+    if (j < nx) {
+        double rv = f ? -ld_v(f + nu, i, j, ny, nx) : -0.0;
+        {
+            double eH = E(i - 1, j) * ld_c(H, i - 1, j, ny, nx);
+            SigN s = sigma_yy(ld_u(u, i - 1, j, ny, nx), ld_u(u, i - 1, j + 1, ny, nx), ld_v(v, i - 1, j, ny, nx),
+                              ld_v(v, i, j, ny, nx), eH, dx_inv, i - 1, ny);
+            rv += s.res * dx_inv;
+        }
+        {
+            double eH = E(i, j) * ld_c(H, i, j, ny, nx);
+            SigN s = sigma_yy(ld_u(u, i, j, ny, nx), ld_u(u, i, j + 1, ny, nx), ld_v(v, i, j, ny, nx),
+                              ld_v(v, i + 1, j, ny, nx), eH, dx_inv, i, ny);
+            rv -= s.res * dx_inv;
+        }
+        {
+            double eH = etaH_vertex(E(i - 1, j - 1), E(i - 1, j), E(i, j - 1), E(i, j), ld_c(H, i - 1, j - 1, ny, nx),
+                                    ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx));
+            SigS s = sigma_xy(ld_u(u, i - 1, j, ny, nx), ld_u(u, i, j, ny, nx), ld_v(v, i, j - 1, ny, nx),
+                              ld_v(v, i, j, ny, nx), eH, dx_inv, i, j, ny, nx);
+            rv -= s.res * dx_inv;
+        }
+        {
+            double eH = etaH_vertex(E(i - 1, j), E(i - 1, j + 1), E(i, j), E(i, j + 1), ld_c(H, i - 1, j, ny, nx),
+                                    ld_c(H, i - 1, j + 1, ny, nx), ld_c(H, i, j, ny, nx), ld_c(H, i, j + 1, ny, nx));
+            SigS s = sigma_xy(ld_u(u, i - 1, j + 1, ny, nx), ld_u(u, i, j + 1, ny, nx), ld_v(v, i, j, ny, nx),
+                              ld_v(v, i, j + 1, ny, nx), eH, dx_inv, i, j + 1, ny, nx);
+            rv += s.res * dx_inv;
+        }
+        {
+            TauB t = tau_b(ld_v(v, i, j, ny, nx), ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j, ny, nx),
+                           ld_c(bed, i - 1, j, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i - 1, j, ny, nx),
+                           ld_c(beta, i, j, ny, nx), p);
+            rv += t.res;
+        }
+        {
+            TauD t = tau_dy(ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j, ny, nx), ld_c(bed, i - 1, j, ny, nx),
+                            ld_c(bed, i, j, ny, nx), dx_inv, i, ny);
+            rv -= t.res;
+        }
+        size_t k = nu + (size_t)i * nx + j;
+        r[k] = sub ? rv - sub[k] : rv;
+    }
+}
+
+template <class EtaFn>
+__device__ __forceinline__ CellNbhd load_nbhd(int i, int j, int ny, int nx, size_t nu, size_t nv,
+                                              const double *__restrict__ u, const double *__restrict__ v,
+                                              const double *__restrict__ H, const double *__restrict__ f,
+                                              const double *__restrict__ bed, const double *__restrict__ beta,
+                                              const EtaFn &E)
+{
+    CellNbhd N;
+    N.e_c = E(i, j); N.e_l = E(i, j - 1); N.e_r = E(i, j + 1); N.e_t = E(i - 1, j); N.e_b = E(i + 1, j);
+    N.e_tl = E(i - 1, j - 1); N.e_tr = E(i - 1, j + 1); N.e_bl = E(i + 1, j - 1); N.e_br = E(i + 1, j + 1);
+    N.H_l = ld_c(H, i, j - 1, ny, nx); N.H_r = ld_c(H, i, j + 1, ny, nx);
+    N.H_t = ld_c(H, i - 1, j, ny, nx); N.H_b = ld_c(H, i + 1, j, ny, nx);
+    N.H_tl = ld_c(H, i - 1, j - 1, ny, nx); N.H_tr = ld_c(H, i - 1, j + 1, ny, nx);
+    N.H_bl = ld_c(H, i + 1, j - 1, ny, nx); N.H_br = ld_c(H, i + 1, j + 1, ny, nx);
+    N.bed_c = ld_c(bed, i, j, ny, nx); N.bed_l = ld_c(bed, i, j - 1, ny, nx); N.bed_r = ld_c(bed, i, j + 1, ny, nx);
+    N.bed_t = ld_c(bed, i - 1, j, ny, nx); N.bed_b = ld_c(bed, i + 1, j, ny, nx);
+    N.beta_c = ld_c(beta, i, j, ny, nx); N.beta_l = ld_c(beta, i, j - 1, ny, nx);
+    N.beta_r = ld_c(beta, i, j + 1, ny, nx); N.beta_t = ld_c(beta, i - 1, j, ny, nx);
+    N.beta_b = ld_c(beta, i + 1, j, ny, nx);
+    N.u_ll = ld_u(u, i, j - 1, ny, nx); N.u_rr = ld_u(u, i, j + 2, ny, nx);
+    N.u_tl = ld_u(u, i - 1, j, ny, nx); N.u_tr = ld_u(u, i - 1, j + 1, ny, nx);
+    N.u_bl = ld_u(u, i + 1, j, ny, nx); N.u_br = ld_u(u, i + 1, j + 1, ny, nx);
+    N.v_lt = ld_v(v, i, j - 1, ny, nx); N.v_lb = ld_v(v, i + 1, j - 1, ny, nx);
+    N.v_rt = ld_v(v, i, j + 1, ny, nx); N.v_rb = ld_v(v, i + 1, j + 1, ny, nx);
+    N.v_tt = ld_v(v, i - 1, j, ny, nx); N.v_bb = ld_v(v, i + 2, j, ny, nx);
+    if (f) {
+        N.f0 = ld_u(f, i, j, ny, nx); N.f1 = ld_u(f, i, j + 1, ny, nx);
+    N.f2 = ld_v(f + nu, i, j, ny, nx); N.f3 = ld_v(f + nu, i + 1, j, ny, nx);
+    N.f4 = ld_v(f + nu + nv, i, j, ny, nx);
+    } else {
+        N.f0 = N.f1 = N.f2 = N.f3 = N.f4 = 0.0;
+    }
+
+    return N;
+}
+
+// ---------------------------------------------------------------------------
+// Forward Vanka: per-cell damped Newton (glide/cuda/vanka.cu:3-439, 844-1295)
+// cd (5 x nc, structure of arrays) receives 0.5*(x - x0) for the four faces
+// and (H - H0) for the cell.
+// ---------------------------------------------------------------------------
+template <int TX, int TY, bool LOCAL>
+__device__ __forceinline__ void vanka_tile(const GlideGeom &g, const Params &p, const double *__restrict__ U,
+                                           const double *__restrict__ f, const double *__restrict__ bed,
+                                           const double *__restrict__ B, const double *__restrict__ beta,
+                                           const double *__restrict__ gamma, const unsigned char *__restrict__ blk,
+                                           int nbx, int n_newton, double *__restrict__ cd, int bx, int by,
+                                           double (*eta_s)[TX + 2])
+{
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv;
+    const int i0 = by * TY, j0 = bx * TX;
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    bool mine = (i < ny && j < nx);
+    if (LOCAL) {
+        mine = mine && blk[(i / RB) * nbx + j / RB];
+        if (!__syncthreads_or(mine)) return;   // uniform: the whole tile is idle
+    }
+    fill_eta_tile<TX, TY>(eta_s, i0, j0, u, v, B, p, g.dx, ny, nx);
+    __syncthreads();
+    if (!mine) return;
+    auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+
+    CellNbhd N = load_nbhd(i, j, ny, nx, nu, nv, u, v, H, f, bed, beta, E);
+    nbhd_grounding(N, p);
+    const double u_l0 = ld_u(u, i, j, ny, nx), u_r0 = ld_u(u, i, j + 1, ny, nx);
+    const double v_t0 = ld_v(v, i, j, ny, nx), v_b0 = ld_v(v, i + 1, j, ny, nx);
+    const double H_c0 = ld_c(H, i, j, ny, nx), th = ld_c(gamma, i, j, ny, nx);
+    double u_l = u_l0, u_r = u_r0, v_t = v_t0, v_b = v_b0, H_c = H_c0;
+    for (int k = 0; k < n_newton; k++) {
+        double J[25], r[5], d[5];
+        vanka_assemble<0>(N, p, i, j, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, false, J, r);
+        lu5_solve(J, r, d);
+        u_l -= 0.25 * d[0];
+        u_r -= 0.25 * d[1];
+        v_t -= 0.25 * d[2];
+        v_b -= 0.25 * d[3];
+        H_c -= 0.25 * d[4];
+        H_c = fmax(H_c, th);
+    }
+    const size_t ci = (size_t)i * nx + j;
+    cd[ci] = 0.5 * (u_l - u_l0);
+    cd[nc + ci] = 0.5 * (u_r - u_r0);
+    cd[2 * nc + ci] = 0.5 * (v_t - v_t0);
+    cd[3 * nc + ci] = 0.5 * (v_b - v_b0);
+    cd[4 * nc + ci] = H_c - H_c0;
+}
+
+// Gather the per-cell updates: U += 0.5 * delta (glide/grid.py:305-317)
+template <int TX, int TY, bool LOCAL>
+__device__ __forceinline__ void apply_tile(int ny, int nx, double *__restrict__ U, const double *__restrict__ cd,
+                                           const unsigned char *__restrict__ blk, int nbx, int bx, int by)
+{
+    const int i = by * TY + threadIdx.y, j = bx * TX + threadIdx.x;
+    if (i > ny || j > nx) return;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
+    auto act = [&](int ci, int cj) { return !LOCAL || blk[(ci / RB) * nbx + cj / RB] != 0; };
+    if (i < ny) {
+        double d = 0.0;
+        if (j > 0 && act(i, j - 1)) d = d + cd[nc + (size_t)i * nx + j - 1];
+        if (j < nx && act(i, j)) d = d + cd[(size_t)i * nx + j];
+        U[(size_t)i * (nx + 1) + j] += 0.5 * d;
+    }
+    if (j < nx) {
+        double d = 0.0;
+        if (i > 0 && act(i - 1, j)) d = d + cd[3 * nc + (size_t)(i - 1) * nx + j];
+        if (i < ny && act(i, j)) d = d + cd[2 * nc + (size_t)i * nx + j];
+        U[nu + (size_t)i * nx + j] += 0.5 * d;
+    }
+    if (i < ny && j < nx) {
+        size_t ci = (size_t)i * nx + j;
+        double d = act(i, j) ? cd[4 * nc + ci] : 0.0;
+        U[nu + nv + ci] += 0.5 * d;
+    }
+}
+
+// Local-smoother activation: one 16x16 workgroup per 14x14 reference block
+// reads its 16x16 footprint (glide/solver.py:148-154, glide/cuda/vanka.cu:870-888).
+__device__ __forceinline__ void activation_block(int ny, int nx, const double *__restrict__ r,
+                                                 const double *__restrict__ U, const double *__restrict__ gamma,
+                                                 unsigned char *__restrict__ blk, int bx, int by, int nbx)
+{
+    const int i = by * RB - 1 + (int)threadIdx.y, j = bx * RB - 1 + (int)threadIdx.x;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    int bad = 0;
+    if (i >= 0 && i < ny && j >= 0 && j < nx) {
+        size_t ci = (size_t)i * nx + j;
+        double a = r[nu + nv + ci], b = U[nu + nv + ci] - gamma[ci];
+        double fb = a + b - sqrt(a * a + b * b);
+        double te = fabs(r[(size_t)i * (nx + 1) + j + 1]) + fabs(r[(size_t)i * (nx + 1) + j]) +
+                    fabs(r[nu + (size_t)(i + 1) * nx + j]) + fabs(r[nu + (size_t)i * nx + j]) + fabs(fb);
+        bad = te > 0.01;
+    }
+    int any = __syncthreads_or(bad);
+    if (threadIdx.x == 0 && threadIdx.y == 0) blk[by * nbx + bx] = (unsigned char)(any != 0);
+}
+
+// ---------------------------------------------------------------------------
+// Bilinear prolongation samples (glide/cuda/utility.cu:180-337)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double bilin(double ty, double tx, double v00, double v01, double v10, double v11)
+{
+    return (1.0 - ty) * ((1.0 - tx) * v00 + tx * v01) + ty * ((1.0 - tx) * v10 + tx * v11);
+}
+// coarse sample helper: value of (a[k] - b[k]) when b != NULL, else a[k]
+__device__ __forceinline__ double cdiff(const double *__restrict__ a, const double *__restrict__ b, size_t k)
+{
+    return b ? a[k] - b[k] : a[k];
+}
+// bilinear u prolongation of (uc - wc) at fine face (j, i) (utility.cu:180-232)
+__device__ __forceinline__ double prol_u_at(const double *uc, const double *wc, int j, int i, int nyc, int nxc)
+{
+    double Jf = (j - 0.5) * 0.5, If = i * 0.5;
+    Jf = fmax(0.0, fmin(Jf, (double)(nyc - 1)));
+    If = fmax(0.0, fmin(If, (double)nxc));
+    int Jl = (int)Jf, Il = (int)If;
+    int Jh = min(Jl + 1, nyc - 1), Ih = min(Il + 1, nxc);
+    double ty = Jf - Jl, tx = If - Il;
+    size_t s = nxc + 1;
+    return bilin(ty, tx, cdiff(uc, wc, Jl * s + Il), cdiff(uc, wc, Jl * s + Ih), cdiff(uc, wc, Jh * s + Il),
+                 cdiff(uc, wc, Jh * s + Ih));
+}
+// utility.cu:235-287
+__device__ __forceinline__ double prol_v_at(const double *vc, const double *wc, int j, int i, int nyc, int nxc)
+{
+    double Jf = j * 0.5, If = (i - 0.5) * 0.5;
+    Jf = fmax(0.0, fmin(Jf, (double)nyc));
+    If = fmax(0.0, fmin(If, (double)(nxc - 1)));
+    int Jl = (int)Jf, Il = (int)If;
+    int Jh = min(Jl + 1, nyc), Ih = min(Il + 1, nxc - 1);
+    double ty = Jf - Jl, tx = If - Il;
+    size_t s = nxc;
+    return bilin(ty, tx, cdiff(vc, wc, Jl * s + Il), cdiff(vc, wc, Jl * s + Ih), cdiff(vc, wc, Jh * s + Il),
+                 cdiff(vc, wc, Jh * s + Ih));
+}
+// utility.cu:289-337
+__device__ __forceinline__ double prol_H_at(const double *hc, const double *wc, int j, int i, int nyc, int nxc)
+{
+    double Jf = (j - 0.5) * 0.5, If = (i - 0.5) * 0.5;
+    Jf = fmax(0.0, fmin(Jf, (double)(nyc - 1)));
+    If = fmax(0.0, fmin(If, (double)(nxc - 1)));
+    int Jl = (int)Jf, Il = (int)If;
+    int Jh = min(Jl + 1, nyc - 1), Ih = min(Il + 1, nxc - 1);
+    double ty = Jf - Jl, tx = If - Il;
+    size_t s = nxc;
+    return bilin(ty, tx, cdiff(hc, wc, Jl * s + Il), cdiff(hc, wc, Jl * s + Ih), cdiff(hc, wc, Jh * s + Il),
+                 cdiff(hc, wc, Jh * s + Ih));
+}
+
+}  // namespace glide

@@ -1,0 +1,180 @@
+"""Batched ensembles of independent ice-sheet solves on one GPU.
+
+Each member is an :class:`IcePhysics` of its own size, level count, geometry
+and basal friction.  One ``Ensemble.forward`` runs the FASCD time step of all
+members together: every operation of the V-cycle is a single persistent
+launch over all members' tiles (libglide_hip.so, ``glide_batch_*``), so the
+launch count per V-cycle does not grow with the member count and the
+latency-bound coarse levels of many members share the GPU.  A member's
+trajectory equals the one ``IcePhysics.forward`` gives it alone, bit for bit.
+
+The reference runs one domain per process and has no batched path; the
+per-member semantics follow glide/physics.py:141-209 (forward) and
+glide/solver.py:71-157 (V-cycle).
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .physics import IcePhysics
+
+BK_VANKA, BK_VANKA_LOCAL, BK_RESIDUAL, BK_RESIDUAL_CHILD = 0, 1, 2, 3
+
+
+@dataclass
+class EnsembleStats:
+    """Outcome of one ensemble time step."""
+    vcycles: list = field(default_factory=list)      # V-cycles run per member
+    latency_s: list = field(default_factory=list)    # wall time until each member finished
+    history: list = field(default_factory=list)      # relative residual after each V-cycle (tol mode)
+    converged: list = field(default_factory=list)
+    wall_s: float = 0.0
+
+
+class Ensemble:
+    """A batch plan over ``members`` (list of IcePhysics on one device)."""
+
+    def __init__(self, members: list[IcePhysics]):
+        if not members:
+            raise ValueError("an ensemble needs at least one member")
+        dev = members[0].device
+        if any(m.device != dev for m in members):
+            raise ValueError("all members must live on one device")
+        self.members = list(members)
+        self.device = dev
+        self.n = len(members)
+        descs = [m._levels() for m in members]
+        self._descs = descs          # kept alive: the plan copies them, but keep for clarity
+        ptrs = (ctypes.POINTER(_lib.GlideLevel) * self.n)(
+            *[ctypes.cast(d, ctypes.POINTER(_lib.GlideLevel)) for d in descs])
+        nlev = (ctypes.c_int * self.n)(*[m.n_levels for m in members])
+        h = ctypes.c_void_p()
+        _lib.call("glide_batch_create", ptrs, nlev, self.n, ctypes.byref(h))
+        self._h = h
+        n_ph = ctypes.c_int()
+        ent = (ctypes.c_int * 32)()
+        cells = (ctypes.c_int * 32)()
+        _lib.call("glide_batch_info", self._h, ctypes.byref(n_ph), ent, cells)
+        self.n_phases = n_ph.value
+        self.entries_per_phase = list(ent[: self.n_phases])
+        self.cells_per_phase = list(cells[: self.n_phases])
+        self.tiles: dict = {}
+        self.schedule = None
+        self.thklim = members[0].thklim
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.call("glide_batch_destroy", self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ----------------------------------------------------------- helpers
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def set_schedule(self, pre=10, post=10, coarse=200, local_passes=10, local_sweeps=10, n_newton=10,
+                     adj_sweeps=10, adj_coarse=100):
+        self.schedule = _lib.GlideSchedule(pre, post, coarse, local_passes, local_sweeps, n_newton, adj_sweeps,
+                                           adj_coarse)
+
+    def set_enabled(self, on):
+        arr = (ctypes.c_ubyte * self.n)(*[1 if x else 0 for x in on])
+        _lib.call("glide_batch_set_enabled", self._h, arr, self._stream())
+
+    def set_tiles(self, kernel: int, phase: int, ty: int, wg_per_cu: int = 0):
+        _lib.call("glide_batch_set_tiles", self._h, int(kernel), int(phase), int(ty), int(wg_per_cu))
+        self.tiles[(int(kernel), int(phase))] = (int(ty), int(wg_per_cu))
+
+    def time_kernel(self, kernel: int, phase: int, ty: int, wg_per_cu: int = 0, reps: int = 5) -> float:
+        ms = ctypes.c_double()
+        _lib.call("glide_batch_time_kernel", self._h, int(kernel), int(phase), int(ty), int(wg_per_cu), int(reps),
+                  ctypes.byref(ms), self._stream())
+        return ms.value
+
+    # ----------------------------------------------------------- solve
+    def forward_setup(self, dt: float):
+        for m in self.members:
+            for g in m.grids:
+                g.dt = float(dt)
+        _lib.call("glide_batch_forward_setup", self._h, float(dt), self._stream())
+
+    def vcycle(self):
+        sched = ctypes.byref(self.schedule) if self.schedule is not None else None
+        _lib.call("glide_batch_vcycle", self._h, float(self.thklim), sched, self._stream())
+
+    def residual_norms(self) -> np.ndarray:
+        out = (ctypes.c_double * self.n)()
+        _lib.call("glide_batch_residual_norms", self._h, out, self._stream())
+        return np.array(out[:], dtype=np.float64)
+
+    def residual(self, phase: int | None = None):
+        """r = F(U) - f on every entry of ``phase`` (default: every finest level)."""
+        _lib.call("glide_batch_residual", self._h, self.n_phases if phase is None else int(phase), self._stream())
+
+    def forward(self, dt: float, n_vcycles: int = 3, tol: float | None = None, max_vcycles: int | None = None,
+                update_geometry: bool = True) -> EnsembleStats:
+        """One implicit time step of every member (glide/physics.py:141-209).
+
+        Without ``tol`` every member runs ``n_vcycles`` V-cycles.  With ``tol``
+        members iterate until their relative residual reaches ``tol`` (at most
+        ``max_vcycles``); a converged member is disabled, so the remaining
+        launches skip its tiles."""
+        st = EnsembleStats()
+        self.forward_setup(dt)
+        on = [True] * self.n
+        self.set_enabled(on)
+        torch.cuda.synchronize(self.device)
+        t0 = time.perf_counter()
+        if tol is None:
+            for _ in range(n_vcycles):
+                self.vcycle()
+            torch.cuda.synchronize(self.device)
+            wall = time.perf_counter() - t0
+            st.vcycles = [n_vcycles] * self.n
+            st.latency_s = [wall] * self.n
+            st.converged = [None] * self.n
+            st.history = [[] for _ in range(self.n)]
+        else:
+            n_max = max_vcycles or max(n_vcycles, 50)
+            r0 = self.residual_norms()
+            st.vcycles = [0] * self.n
+            st.latency_s = [None] * self.n
+            st.converged = [False] * self.n
+            st.history = [[] for _ in range(self.n)]
+            for it in range(n_max):
+                self.vcycle()
+                r = self.residual_norms()
+                t = time.perf_counter() - t0
+                for m in range(self.n):
+                    if not on[m]:
+                        continue
+                    rel = r[m] / r0[m] if r0[m] > 0 else 0.0
+                    st.history[m].append(float(rel))
+                    st.vcycles[m] = it + 1
+                    if rel <= tol:
+                        on[m] = False
+                        st.converged[m] = True
+                        st.latency_s[m] = t
+                if not any(on):
+                    break
+                self.set_enabled(on)
+            t = time.perf_counter() - t0
+            st.latency_s = [x if x is not None else t for x in st.latency_s]
+            self.set_enabled([True] * self.n)
+            wall = t
+        st.wall_s = wall
+        if update_geometry:
+            for m in self.members:
+                m.grid.H_prev.copy_(m.grid.H)
+        return st

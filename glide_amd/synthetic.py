@@ -112,3 +112,24 @@ def ice_slab(ny: int, nx: int, dx: float, seed: int, smb: float = 0.3) -> dict:
     return {"bed": np.ascontiguousarray(np.broadcast_to(bed, (ny, nx))), "H": np.ascontiguousarray(np.broadcast_to(H, (ny, nx))),
             "beta": 10.0 ** logb / RHO_G, "B": np.full((ny, nx), B_HEAD), "smb": np.full((ny, nx), float(smb)),
             "dx": float(dx)}
+
+
+# Batched ensemble: 16 members each at 256^2, 384 x 512, 512^2 and 768^2,
+# seeds 1000-1063 (in that order), dx = 1500 m, the dome / bed / beta
+# generators of the 128^2 workload (radius 100/256 of the smaller side), and
+# the largest level count that keeps the coarsest side >= 16.
+ENSEMBLE_SIZES = [(256, 256)] * 16 + [(384, 512)] * 16 + [(512, 512)] * 16 + [(768, 768)] * 16
+ENSEMBLE_DX = 1500.0
+ENSEMBLE_SEED0 = 1000
+
+
+def ensemble_members(sizes=None, seed0: int = ENSEMBLE_SEED0, dx: float = ENSEMBLE_DX) -> list[dict]:
+    """Inputs of each ensemble member: ice_cap(ny, nx, dx, seed) plus its
+    level count (max_levels) and seed."""
+    sizes = ENSEMBLE_SIZES if sizes is None else sizes
+    out = []
+    for k, (ny, nx) in enumerate(sizes):
+        d = ice_cap(ny, nx, dx, seed0 + k)
+        d.update(ny=ny, nx=nx, seed=seed0 + k, n_levels=max_levels(ny, nx))
+        out.append(d)
+    return out

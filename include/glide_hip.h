@@ -193,6 +193,63 @@ int glide_compute_mask(const GlideLevel *level, void *stream);
  * (glide/solver.py:187-189, glide/grid.py:322-324). */
 int glide_adjoint_residual(const GlideLevel *level, void *stream);
 
+/* ---- batched ensembles ------------------------------------------------
+ * An ensemble of independent members, each a level hierarchy of its own
+ * size (GlideLevel arrays as above, finest first), solved together: every
+ * operation of the V-cycle is one persistent launch over all members' tiles.
+ * Member m's level k runs in phase k + (L_max - L_m): hierarchies are aligned
+ * at their coarsest level.  The reference has no batched path: a member's
+ * results equal those of glide_fascd_vcycle / glide_forward_setup /
+ * glide_residual_norm applied to that member alone (bitwise).  Not
+ * thread-safe per plan; distinct plans may run on distinct streams. */
+typedef struct GlideBatch GlideBatch;
+
+/* Kernel ids for the tile autotuner (glide_batch_set_tiles / _time_kernel). */
+#define GLIDE_BK_VANKA 0          /* full Vanka sweep, tile height 1/2/4/8 */
+#define GLIDE_BK_VANKA_LOCAL 1    /* local Vanka sweep, tile height 1/2/4/8 */
+#define GLIDE_BK_RESIDUAL 2       /* residual, tile height 2/4/8/16 */
+#define GLIDE_BK_RESIDUAL_CHILD 3 /* FAS right-hand side on the child level */
+
+/* Creates the plan: copies the member descriptors, builds the tile tables and
+ * allocates the plan's own small device tables and pinned staging (the only
+ * allocating batched call).  members[m] points to n_levels[m] levels. */
+int glide_batch_create(const GlideLevel *const *members, const int *n_levels, int n_members,
+                       GlideBatch **out);
+int glide_batch_destroy(GlideBatch *batch);
+
+/* Phase count, entries (members present) and cells per phase. */
+int glide_batch_info(const GlideBatch *batch, int *n_phases, int *entries_per_phase,
+                     int *cells_per_phase);
+
+/* dt on every level of every member and f_H = H_prev / dt + smb on each
+ * finest level: glide_forward_setup for all members (glide/physics.py:166-173). */
+int glide_batch_forward_setup(GlideBatch *batch, double dt, void *stream);
+
+/* One FASCD V-cycle of every enabled member (glide/solver.py:71-157). */
+int glide_batch_vcycle(GlideBatch *batch, double thklim, const GlideSchedule *sched,
+                       void *stream);
+
+/* Per-member ||(r_u, r_v, FB(r_H, H - gamma))||_2 of the finest level into
+ * out_host[n_members] (glide/physics.py:176-203); synchronises the stream. */
+int glide_batch_residual_norms(GlideBatch *batch, double *out_host, void *stream);
+
+/* r = F(U) - f on every entry of one phase (phase n_phases = every member's
+ * finest level): the ensemble-wide residual stencil sweep. */
+int glide_batch_residual(GlideBatch *batch, int phase, void *stream);
+
+/* on[n_members]: members with on[m] == 0 are skipped by every batched call
+ * (converged members stop iterating); synchronises the stream. */
+int glide_batch_set_enabled(GlideBatch *batch, const unsigned char *on, void *stream);
+
+/* Autotuner hooks: tile height ty and workgroups per CU (0 = register-
+ * budget default) of one kernel in one phase; _time_kernel runs it reps
+ * times with those settings (state-preserving: the Vanka timing writes only
+ * scratch, the residual timings only r and the child's f) and returns the
+ * mean time per launch in ms, leaving the plan's settings unchanged. */
+int glide_batch_set_tiles(GlideBatch *batch, int kernel, int phase, int ty, int wg_per_cu);
+int glide_batch_time_kernel(GlideBatch *batch, int kernel, int phase, int ty, int wg_per_cu,
+                            int reps, double *ms, void *stream);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *glide_last_error(void);
 

@@ -48,3 +48,34 @@ def test_errors_are_reported_without_a_gpu():
         _lib.call("glide_residual", ctypes.byref(g), ctypes.byref(p), 1, 1, None, None, 1, 1, 1, 1, 1, None)
     assert e.value.code == _lib.GLIDE_E_ARG
     assert _lib.load().glide_version().startswith(b"glide_amd")
+
+
+def test_batch_errors_are_reported_without_a_gpu():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    out = ctypes.c_void_p()
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_batch_create", None, None, 1, ctypes.byref(out))
+    assert e.value.code == _lib.GLIDE_E_NULL
+    # a member whose level 1 is not a 2x coarsening of level 0
+    lv = (_lib.GlideLevel * 2)()
+    for k, n in enumerate((16, 6)):
+        lv[k].g = _lib.GlideGeom(n, n, 1.0, 1.0)
+        for name in ("U", "f", "r", "w", "bed", "B", "beta", "gamma", "chi", "mask", "scratch", "H_prev", "smb"):
+            setattr(lv[k], name, 8)   # never dereferenced: validation fails first
+        lv[k].active = 8
+    ptrs = (ctypes.POINTER(_lib.GlideLevel) * 1)(ctypes.cast(lv, ctypes.POINTER(_lib.GlideLevel)))
+    nlev = (ctypes.c_int * 1)(2)
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_batch_create", ptrs, nlev, 1, ctypes.byref(out))
+    assert e.value.code == _lib.GLIDE_E_SIZE and "coarsening" in str(e.value)
+    nlev[0] = 0
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_batch_create", ptrs, nlev, 1, ctypes.byref(out))
+    assert e.value.code == _lib.GLIDE_E_SIZE
+    for name, args in (("glide_batch_vcycle", (None, 0.1, None, None)),
+                       ("glide_batch_set_tiles", (None, 0, 0, 4, 0)),
+                       ("glide_batch_forward_setup", (None, 1.0, None))):
+        with pytest.raises(_lib.GlideError) as e:
+            _lib.call(name, *args)
+        assert e.value.code == _lib.GLIDE_E_NULL
