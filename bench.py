@@ -1,16 +1,24 @@
-"""Benchmark of the forward step of the FP64 SSA / FASCD solver on one MI355X.
+"""Benchmark of the FP64 SSA / FASCD forward step on one MI355X.
 
-Workload cap_512: 512 x 512 MAC grid, dx = 1500 m, 5 levels, dt = 10 a,
-3 FASCD V-cycles per step (the reference's step, glide/physics.py:141-209),
-seeded synthetic ice cap with a floating margin (glide_amd/synthetic.py).
+Headline (the "value"): ms per forward step to 1e-8 relative residual at
+512 x 512, dx = 1500 m, 5 levels, dt = 10 a, on seeded ice sheets
+(glide_amd/synthetic.py ice_sheet, test seeds 20000...), starting from
+u = v = 0 with H = H_prev, iterating FASCD V-cycles until the reference's
+residual norm (glide/physics.py:176-203) is <= 1e-8 x the cold start's
+initial norm.  Plain FASCD and the learned warm start (the trained network
+of glide_amd/data/warmstart_unet.npz; inference inside the timed region)
+are timed on the same seeds; value = the learned time (plain if no weights).
+One "step" = one seed's forward step; inputs are resident in HBM.
 
-One "step" = IcePhysics.forward(dt, n_vcycles=3) on inputs already resident in
-HBM.  Prints ONE JSON line.  Besides the step time it reports:
-  - the relative residual after the timed steps and whether 1e-8 is reached;
-  - roofline: the dominant kernel (the finest-level Vanka sweep, FP64
-    arithmetic) timed with HIP events on the solver's stream, priced with the
-    algorithmic flop count per cell-iteration frozen below;
-  - stencil_roofline: the residual stencil (HBM) the same way;
+Also reported on the same JSON line:
+  - learned_batched: 16 such sheets as one ensemble, plain vs learned;
+  - reference_schedule: the parity workload cap_512 (floating margin) on the
+    reference's fixed 3-V-cycle step;
+  - roofline: the finest-level Vanka sweep (FP64 VALU), HIP-event timed on
+    the solver's stream, priced with the frozen per-cell flop count;
+  - stencil_roofline / ensemble: the residual stencil (HBM), single domain
+    and over the 64-member ensemble, whose batched step is timed against
+    solving the members one by one;
   - cpu_baseline: the CPU oracle (glide_oracle.c, OpenMP) on a bounded sample.
 """
 from __future__ import annotations
@@ -31,8 +39,8 @@ from glide_amd.roofline import (FLOPS_NEWTON_ITER, PEAK_FP64_TFLOPS, PEAK_HBM_GB
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=4, help="timed forward steps (one test seed each)")
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cap_512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -193,56 +201,203 @@ def ensemble_bench(dt=10.0, n_vcycles=3):
     return out
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU is out of scope)"}))
-        sys.exit(2)
+WEIGHTS = os.path.join(ROOT, "glide_amd", "data", "warmstart_unet.npz")
+TOL = 1e-8
+N512 = 512
+DX512 = 1500.0
+LEVELS512 = 5
+
+
+def to_tolerance_bench(steps: int, warmup: int, dt=10.0):
+    """One forward step to ||r|| <= 1e-8 ||r0_cold|| on seeded 512^2 ice
+    sheets (test seeds 20000...), plain FASCD from u = v = 0 against the
+    learned warm start (network inference included in the timed region)."""
+    import torch
+
+    from glide_amd.train_warmstart import TEST_SEED0, member
+    from glide_amd.warmstart import UNet, WarmStart
+
+    net = UNet.load(WEIGHTS) if os.path.exists(WEIGHTS) else None
+    ws = WarmStart(net) if net is not None else None
+    plain = WarmStart(None)
+    rows, last = [], None
+    for k in range(warmup + steps):
+        seed = TEST_SEED0 + k
+        ph = member(seed, N512, dx=DX512, n_levels=LEVELS512)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cold = plain.solve(ph, dt, tol=TOL, max_vcycles=300, use_net=False)
+        torch.cuda.synchronize()
+        t_plain = (time.perf_counter() - t0) * 1e3
+        row = {"seed": seed, "plain_ms": t_plain, "vcycles_plain": cold["vcycles"], "plain_converged": cold["converged"]}
+        if ws is not None:
+            ph.grid.u.zero_()
+            ph.grid.v.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            warm = ws.solve(ph, dt, tol=TOL, max_vcycles=300, use_net=True)
+            torch.cuda.synchronize()
+            row.update(learned_ms=(time.perf_counter() - t0) * 1e3, vcycles_learned=warm["vcycles"],
+                       learned_converged=warm["converged"], fallback=warm["fallback"],
+                       r0_warm_over_cold=warm["r0_warm"] / cold["r0_cold"])
+        if k >= warmup:
+            rows.append(row)
+        if last is not None:
+            del last
+        last = ph
+    return rows, last, net
+
+
+def learned_batched_bench(net, n_members=16, dt=10.0):
+    """16 seeded 512^2 sheets as one ensemble: plain batched FASCD vs batched
+    network inference + batched FASCD, both to 1e-8 of each member's cold
+    initial residual."""
+    import ctypes
+
+    import torch
+
+    from glide_amd import _lib
+    from glide_amd.ensemble import Ensemble
+    from glide_amd.train_warmstart import TEST_SEED0, member
+    from glide_amd.warmstart import IN_CH, VEL_SCALE, features
+
+    seeds = [TEST_SEED0 + 100 + k for k in range(n_members)]
+    out = {"members": n_members, "size": f"{N512}x{N512}"}
+    for mode in ("plain", "learned"):
+        if mode == "learned" and net is None:
+            break
+        mem = [member(s, N512, dx=DX512, n_levels=LEVELS512) for s in seeds]
+        ens = Ensemble(mem)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.forward_setup(dt)
+        r0 = ens.residual_norms()
+        if mode == "learned":
+            X = torch.empty((n_members, N512 * N512 * IN_CH), dtype=torch.int16, device=mem[0].device)
+            for k, ph in enumerate(mem):
+                features(ph, out=X[k])
+            pred = net.forward(X, n_members, N512, N512)
+            st_ = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            for k, ph in enumerate(mem):
+                g = ph.grid
+                _lib.call("glide_unet_to_faces", ctypes.c_void_p(pred[k * N512 * N512 * 2:].data_ptr()), g.ny, g.nx,
+                          VEL_SCALE, g.u.data_ptr(), g.v.data_ptr(), st_)
+            rw = ens.residual_norms()
+            for k, ph in enumerate(mem):       # per-member fallback
+                if not rw[k] <= r0[k]:
+                    ph.grid.u.zero_()
+                    ph.grid.v.zero_()
+        st = ens.forward(dt, tol=TOL, max_vcycles=300, update_geometry=False, r0=r0, setup=False)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) * 1e3
+        out[mode] = {"wall_ms": round(wall, 1), "ms_per_member_step": round(wall / n_members, 2),
+                     "vcycles_mean": float(sum(st.vcycles) / n_members), "vcycles_max": int(max(st.vcycles)),
+                     "all_converged": bool(all(st.converged))}
+        ens.close()
+        del mem, ens
+        torch.cuda.empty_cache()
+    if "learned" in out:
+        out["speedup_learned_vs_plain"] = round(out["plain"]["wall_ms"] / out["learned"]["wall_ms"], 3)
+    return out
+
+
+def reference_schedule_bench(dt=10.0, steps=3):
+    """The parity workload cap_512 (ice cap with a floating margin) on the
+    reference's fixed step of 3 V-cycles: ms per step and the relative
+    residual reached (plain FASCD stalls near 1e-3 on floating margins)."""
     import torch
 
     from glide_amd import synthetic as S
     from glide_amd.physics import IcePhysics
 
-    w = S.workload(args.workload)
+    w = S.workload("cap_512")
     ph = IcePhysics(w["ny"], w["nx"], w["dx"], n_levels=w["n_levels"])
     ph.set_geometry(w["bed"], w["H"])
     ph.set_parameters(B=w["B"], beta=w["beta"], smb=w["smb"])
+    ph.forward(w["dt"], n_vcycles=w["n_vcycles"])
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ph.forward(w["dt"], n_vcycles=w["n_vcycles"])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    ph.forward(w["dt"], n_vcycles=w["n_vcycles"], tol=1e-300, max_vcycles=w["n_vcycles"])
+    return {"workload": "cap_512: 512x512, dx=1500 m, 5 levels, dt=10 a, 3 V-cycles/step, floating margin",
+            "ms_per_step": round(ms, 3), "rel_residual_after_step": [float(f"{x:.3e}") for x in ph.last_history]}
+
+
+def cpu_baseline_tol(seed: int, vcycles: int, threads: int, dt=10.0):
+    """The CPU oracle (OpenMP) on the headline workload: one V-cycle of the
+    512^2 sheet from the cold start timed, times the plain V-cycle count."""
+    from glide_amd import synthetic as S
+    from oracle import oracle as O
+
+    if threads > 0:
+        O.set_threads(threads)
+    d = S.ice_sheet(N512, N512, DX512, seed)
+    h = O.Hierarchy(N512, N512, DX512, n_levels=LEVELS512)
+    h.set_geometry(d["bed"], d["H"])
+    h.set_parameters(B=d["B"], beta=d["beta"], smb=d["smb"])
+    t0 = time.perf_counter()
+    h.forward(dt, n_vcycles=1, update_geometry=False)
+    one = time.perf_counter() - t0
+    return {"value": round(one * vcycles * 1e3, 1), "unit": "ms per forward step to 1e-8 (plain FASCD)",
+            "cores": O.num_threads(), "kind": "port",
+            "sample": f"1 V-cycle of the 512^2 sheet (seed {seed}) timed ({one:.1f} s), x{vcycles} V-cycles "
+                      f"(the GPU's plain V-cycle count for that step)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU is out of scope)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
     if args.kernels_only:
+        from glide_amd import synthetic as S
+        from glide_amd.physics import IcePhysics
+
+        w = S.workload(args.workload)
+        ph = IcePhysics(w["ny"], w["nx"], w["dx"], n_levels=w["n_levels"])
+        ph.set_geometry(w["bed"], w["H"])
+        ph.set_parameters(B=w["B"], beta=w["beta"], smb=w["smb"])
         ph.forward(w["dt"], n_vcycles=w["n_vcycles"])
         print(json.dumps(kernel_rooflines(ph)))
         return
-    for _ in range(args.warmup):
-        ph.forward(w["dt"], n_vcycles=w["n_vcycles"])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ph.forward(w["dt"], n_vcycles=w["n_vcycles"])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms = elapsed / args.steps * 1e3
-    # relative residual reached by one more step (not timed)
-    ph.forward(w["dt"], n_vcycles=w["n_vcycles"], tol=1e-300, max_vcycles=w["n_vcycles"])
-    hist = ph.last_history
+    rows, ph, net = to_tolerance_bench(args.steps, args.warmup)
+    plain_ms = float(np.mean([r["plain_ms"] for r in rows]))
+    learned = net is not None
+    learned_ms = float(np.mean([r["learned_ms"] for r in rows])) if learned else None
     rl = kernel_rooflines(ph)
+    del ph
+    torch.cuda.empty_cache()
+    value = learned_ms if learned else plain_ms
     line = {
-        "metric": "ms per forward step (3 FASCD V-cycles, FP64) at 512^2; relative residual reached vs 1e-8",
-        "value": round(ms, 3), "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (seeded ice cap, glide_amd/synthetic.py)",
-        "config": {"workload": f"{w['name']}: {w['ny']}x{w['nx']}, dx={w['dx']:.0f} m, {w['n_levels']} levels, "
-                               f"dt={w['dt']:.0f} a, {w['n_vcycles']} V-cycles/step",
+        "metric": "ms per forward step to 1e-8 relative residual at 512^2 (learned warm start vs plain FASCD), FP64",
+        "value": round(value, 3), "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded ice sheets, glide_amd/synthetic.py ice_sheet, test seeds 20000+)",
+        "config": {"workload": f"ice_sheet_512: {N512}x{N512}, dx={DX512:.0f} m, {LEVELS512} levels, dt=10 a, one "
+                               f"forward step from u = v = 0 to ||r|| <= 1e-8 ||r0_cold||"
+                               + (" with the learned warm start" if learned else " (no trained weights: plain)"),
                    "global_batch": 1, "parallelism": "single GPU"},
-        "rel_residual_after_step": [float(f"{x:.3e}") for x in hist],
-        "reaches_1e-8": bool(hist and min(hist) <= 1e-8),
+        "plain_ms_per_step": round(plain_ms, 3),
+        "learned_ms_per_step": round(learned_ms, 3) if learned else None,
+        "speedup_learned_vs_plain": round(plain_ms / learned_ms, 3) if learned else None,
+        "vcycles_plain": [r["vcycles_plain"] for r in rows],
+        "vcycles_learned": [r.get("vcycles_learned") for r in rows],
+        "fallbacks": int(sum(bool(r.get("fallback")) for r in rows)),
+        "all_converged": bool(all(r["plain_converged"] and r.get("learned_converged", True) for r in rows)),
         **rl,
     }
+    line["learned_batched"] = learned_batched_bench(net)
+    line["reference_schedule"] = reference_schedule_bench()
     if not args.no_ensemble:
-        del ph
-        torch.cuda.empty_cache()
         line["ensemble"] = ensemble_bench()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w, args.cpu_threads)
+        line["cpu_baseline"] = cpu_baseline_tol(rows[0]["seed"], rows[0]["vcycles_plain"], args.cpu_threads)
     print(json.dumps(line))
 
 

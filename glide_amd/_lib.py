@@ -86,6 +86,33 @@ SIGNATURES = {
     "glide_batch_set_enabled": (c_int, [c_void_p, POINTER(ctypes.c_ubyte), c_void_p]),
     "glide_batch_set_tiles": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "glide_batch_time_kernel": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, POINTER(c_double), c_void_p]),
+    "glide_unet_conv3x3": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p,
+                                   c_int, c_void_p, c_void_p]),
+    "glide_unet_conv3x3_dgrad": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int,
+                                         c_int, c_void_p, c_int, c_void_p]),
+    "glide_unet_wgrad_chunks": (ctypes.c_long, [c_int, c_int, c_int]),
+    "glide_unet_conv3x3_wgrad": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_int,
+                                         c_void_p, c_void_p, c_void_p]),
+    "glide_unet_gemm_bf16": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "glide_unet_prep_weights": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "glide_unet_relu_bwd": (c_int, [c_void_p, c_void_p, ctypes.c_long, c_void_p, c_void_p]),
+    "glide_unet_pool2": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "glide_unet_pool2_bwd": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p]),
+    "glide_unet_up2": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "glide_unet_up2_bwd": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "glide_unet_bias_chunks": (ctypes.c_long, [ctypes.c_long]),
+    "glide_unet_bias_grad": (c_int, [c_void_p, ctypes.c_long, c_int, c_void_p, c_void_p, c_void_p]),
+    "glide_unet_head": (c_int, [c_void_p, ctypes.c_long, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "glide_unet_head_bwd": (c_int, [c_void_p, c_void_p, ctypes.c_long, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p]),
+    "glide_unet_mse_blocks": (ctypes.c_long, [ctypes.c_long]),
+    "glide_unet_mse": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, ctypes.c_float, c_void_p, c_void_p,
+                               c_void_p]),
+    "glide_unet_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_long, ctypes.c_float,
+                                ctypes.c_float, ctypes.c_float, ctypes.c_float, c_int, ctypes.c_float, c_void_p]),
+    "glide_unet_features": (c_int, [DP, DP, DP, DP, c_int, c_int, c_double, c_void_p, c_void_p]),
+    "glide_unet_targets": (c_int, [DP, DP, c_int, c_int, c_double, c_void_p, c_void_p]),
+    "glide_unet_to_faces": (c_int, [c_void_p, c_int, c_int, c_double, DP, DP, c_void_p]),
     "glide_last_error": (c_char_p, []),
     "glide_version": (c_char_p, []),
 }

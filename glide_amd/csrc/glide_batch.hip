@@ -107,20 +107,20 @@ template <int TY, bool CHILD>
 __global__ __launch_bounds__(BTX *TY) void kb_residual(const BEntry *__restrict__ E, const int *__restrict__ pre, int n,
                                                         const unsigned char *__restrict__ on)
 {
-    __shared__ double eta_s[TY + 2][BTX + 2];
+    __shared__ ResidLDS<BTX, TY> S;
     tile_loop(pre[n], [&](int t) {
         const int e = find_entry(pre, n, t);
         const BEntry &B = E[e];
         if (!on[B.member]) return;
         const GlideLevel &A = CHILD ? B.C : B.L;
         const int lt = t - pre[e], tx = cdiv(A.g.nx + 1, BTX);
-        __syncthreads();   // the previous tile's readers of eta_s are done
+        __syncthreads();   // the previous tile's readers of S are done
         if (CHILD)
-            residual_tile<BTX, TY>(A.g, to_params(A.p), A.f, A.U, nullptr, A.r, A.bed, A.B, A.beta, A.mask, A.gamma,
-                                   lt % tx, lt / tx, eta_s);
+            residual_tile_lds<BTX, TY>(A.g, to_params(A.p), A.f, A.U, nullptr, A.r, A.bed, A.B, A.beta, A.mask,
+                                       A.gamma, lt % tx, lt / tx, S);
         else
-            residual_tile<BTX, TY>(A.g, to_params(A.p), A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma,
-                                   lt % tx, lt / tx, eta_s);
+            residual_tile_lds<BTX, TY>(A.g, to_params(A.p), A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask,
+                                       A.gamma, lt % tx, lt / tx, S);
     });
 }
 

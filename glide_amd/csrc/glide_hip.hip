@@ -72,8 +72,8 @@ __global__ __launch_bounds__(NT) void k_residual(GlideGeom g, Params p, double *
                                                  const double *__restrict__ B, const double *__restrict__ beta,
                                                  const double *__restrict__ mask, const double *__restrict__ gamma)
 {
-    __shared__ double eta_s[TY + 2][TX + 2];
-    residual_tile<TX, TY>(g, p, r, U, f, sub, bed, B, beta, mask, gamma, blockIdx.x, blockIdx.y, eta_s);
+    __shared__ ResidLDS<TX, TY> S;
+    residual_tile_lds<TX, TY>(g, p, r, U, f, sub, bed, B, beta, mask, gamma, blockIdx.x, blockIdx.y, S);
 }
 
 // ---------------------------------------------------------------------------

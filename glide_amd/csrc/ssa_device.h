@@ -65,28 +65,37 @@ __host__ __device__ __forceinline__ Dual operator*(Dual a, double s) { return {a
 __host__ __device__ __forceinline__ Dual operator+(Dual a, double s) { return {a.v + s, a.d}; }
 
 // Cell viscosity eta = 1/2 B (eps_II^2 + eps_reg)^((1-n)/2n) with the four
-// corner shear rates masked at the domain edge (viscosity.cu:61-112).
+// corner shear rates masked at the domain edge (viscosity.cu:61-112).  U(a, b)
+// and V(a, b) return the clamped u[a, b] and v[a, b]; Bc is B at the cell.
+template <class UA, class VA>
+__host__ __device__ __forceinline__ double eta_cell_acc(int i, int j, const UA &U, const VA &V, double Bc,
+                                                        const Params &p, double dx, int ny, int nx)
+{
+    double dx_inv = 1.0 / dx;
+    double glen_exp = (1.0 - p.n) / (2.0 * p.n);
+    double u_l = U(i, j), u_r = U(i, j + 1);
+    double v_t = V(i, j), v_b = V(i + 1, j);
+    double dudx = (u_r - u_l) * dx_inv;
+    double dvdy = (v_t - v_b) * dx_inv;
+    double tl_mask = (i > 0 && j > 0);
+    double e_tl = 0.5 * ((U(i - 1, j) - u_l) * dx_inv + (v_t - V(i, j - 1)) * dx_inv) * tl_mask;
+    double tr_mask = (i > 0 && j < (nx - 1));
+    double e_tr = 0.5 * ((U(i - 1, j + 1) - u_r) * dx_inv + (V(i, j + 1) - v_t) * dx_inv) * tr_mask;
+    double bl_mask = (i < (ny - 1) && j > 0);
+    double e_bl = 0.5 * ((u_l - U(i + 1, j)) * dx_inv + (v_b - V(i + 1, j - 1)) * dx_inv) * bl_mask;
+    double br_mask = (i < (ny - 1) && j < (nx - 1));
+    double e_br = 0.5 * ((u_r - U(i + 1, j + 1)) * dx_inv + (V(i + 1, j + 1) - v_b) * dx_inv) * br_mask;
+    double exy2 = 0.25 * (e_tl * e_tl + e_tr * e_tr + e_bl * e_bl + e_br * e_br);
+    double eII = dudx * dudx + dvdy * dvdy + dudx * dvdy + exy2 + p.eps_reg;
+    return 0.5 * Bc * dm_pow(eII, glen_exp);
+}
 __host__ __device__ __forceinline__ double eta_cell(int i, int j, const double *__restrict__ u,
                                            const double *__restrict__ v, const double *__restrict__ B,
                                            const Params &p, double dx, int ny, int nx)
 {
-    double dx_inv = 1.0 / dx;
-    double glen_exp = (1.0 - p.n) / (2.0 * p.n);
-    double u_l = ld_u(u, i, j, ny, nx), u_r = ld_u(u, i, j + 1, ny, nx);
-    double v_t = ld_v(v, i, j, ny, nx), v_b = ld_v(v, i + 1, j, ny, nx);
-    double dudx = (u_r - u_l) * dx_inv;
-    double dvdy = (v_t - v_b) * dx_inv;
-    double tl_mask = (i > 0 && j > 0);
-    double e_tl = 0.5 * ((ld_u(u, i - 1, j, ny, nx) - u_l) * dx_inv + (v_t - ld_v(v, i, j - 1, ny, nx)) * dx_inv) * tl_mask;
-    double tr_mask = (i > 0 && j < (nx - 1));
-    double e_tr = 0.5 * ((ld_u(u, i - 1, j + 1, ny, nx) - u_r) * dx_inv + (ld_v(v, i, j + 1, ny, nx) - v_t) * dx_inv) * tr_mask;
-    double bl_mask = (i < (ny - 1) && j > 0);
-    double e_bl = 0.5 * ((u_l - ld_u(u, i + 1, j, ny, nx)) * dx_inv + (v_b - ld_v(v, i + 1, j - 1, ny, nx)) * dx_inv) * bl_mask;
-    double br_mask = (i < (ny - 1) && j < (nx - 1));
-    double e_br = 0.5 * ((u_r - ld_u(u, i + 1, j + 1, ny, nx)) * dx_inv + (ld_v(v, i + 1, j + 1, ny, nx) - v_b) * dx_inv) * br_mask;
-    double exy2 = 0.25 * (e_tl * e_tl + e_tr * e_tr + e_bl * e_bl + e_br * e_br);
-    double eII = dudx * dudx + dvdy * dvdy + dudx * dvdy + exy2 + p.eps_reg;
-    return 0.5 * ld_c(B, i, j, ny, nx) * dm_pow(eII, glen_exp);
+    auto U = [&](int a, int b) { return ld_u(u, a, b, ny, nx); };
+    auto V = [&](int a, int b) { return ld_v(v, a, b, ny, nx); };
+    return eta_cell_acc(i, j, U, V, ld_c(B, i, j, ny, nx), p, dx, ny, nx);
 }
 
 // Dual-number viscosity: value and directional derivative along (du, dv)

@@ -169,6 +169,164 @@ __device__ __forceinline__ void residual_tile(const GlideGeom &g, const Params &
     }
 }
 
+// ---------------------------------------------------------------------------
+// Residual, LDS-staged form: the tile's u, v (two-ring halo), H, bed, beta
+// (one-ring halo) are staged once; every cell's viscosity, grounding sigmoid,
+// sigma_xx and sigma_yy and every vertex's sigma_xy are evaluated once into
+// LDS; each point then sums its rows from LDS.  Every quantity is the same
+// expression of the same (clamped) values as in residual_tile, and each row
+// sums its terms in the same order, so the two give identical bits.
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+struct ResidLDS {
+    double u[TY + 4][TX + 4];   // u[i0-2+a][j0-2+b] (clamped)
+    double v[TY + 4][TX + 4];   // v[i0-2+a][j0-2+b] (clamped)
+    double H[TY + 2][TX + 2];   // cells [i0-1+a][j0-1+b] (clamped)
+    double bed[TY + 2][TX + 2];
+    double beta[TY + 2][TX + 2];
+    double E[TY + 2][TX + 2];   // viscosity (0 outside the domain)
+    double g[TY + 2][TX + 2];   // grounding sigmoid and its derivative
+    double gd[TY + 2][TX + 2];
+    double sxx[TY + 2][TX + 2]; // sigma_xx / sigma_yy residual parts
+    double syy[TY + 2][TX + 2];
+    double sxy[TY + 1][TX + 1]; // vertices [i0+a][j0+b]
+};
+
+template <int TX, int TY>
+__device__ __forceinline__ void residual_tile_lds(const GlideGeom &g, const Params &p, double *__restrict__ r,
+                                                  const double *__restrict__ U, const double *__restrict__ f,
+                                                  const double *__restrict__ sub, const double *__restrict__ bed,
+                                                  const double *__restrict__ B, const double *__restrict__ beta,
+                                                  const double *__restrict__ mask, const double *__restrict__ gamma,
+                                                  int bx, int by, ResidLDS<TX, TY> &S)
+{
+    constexpr int NT = TX * TY;
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv;
+    const int i0 = by * TY, j0 = bx * TX;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+    // 1. stage the inputs
+    for (int k = tid; k < (TY + 4) * (TX + 4); k += NT) {
+        const int a = k / (TX + 4), b = k - a * (TX + 4);
+        S.u[a][b] = ld_u(u, i0 - 2 + a, j0 - 2 + b, ny, nx);
+        S.v[a][b] = ld_v(v, i0 - 2 + a, j0 - 2 + b, ny, nx);
+    }
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        const int a = k / (TX + 2), b = k - a * (TX + 2);
+        S.H[a][b] = ld_c(H, i0 - 1 + a, j0 - 1 + b, ny, nx);
+        S.bed[a][b] = ld_c(bed, i0 - 1 + a, j0 - 1 + b, ny, nx);
+        S.beta[a][b] = ld_c(beta, i0 - 1 + a, j0 - 1 + b, ny, nx);
+    }
+    __syncthreads();
+    auto UL = [&](int a, int b) { return S.u[a - i0 + 2][b - j0 + 2]; };   // global (a, b)
+    auto VL = [&](int a, int b) { return S.v[a - i0 + 2][b - j0 + 2]; };
+    auto HC = [&](int a, int b) { return S.H[a - i0 + 1][b - j0 + 1]; };
+    auto EC = [&](int a, int b) { return S.E[a - i0 + 1][b - j0 + 1]; };
+    // 2. per cell: viscosity and grounding sigmoid
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        const int a = k / (TX + 2), b = k - a * (TX + 2);
+        const int ci = i0 - 1 + a, cj = j0 - 1 + b;
+        S.E[a][b] = (ci >= 0 && ci < ny && cj >= 0 && cj < nx)
+                        ? eta_cell_acc(ci, cj, UL, VL, B[(size_t)ci * nx + cj], p, g.dx, ny, nx)
+                        : 0.0;
+        const Ground gr = ground(S.bed[a][b], S.H[a][b], p);
+        S.g[a][b] = gr.g;
+        S.gd[a][b] = gr.gd;
+    }
+    __syncthreads();
+    // 3. per cell normal stresses, per vertex shear stress
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        const int a = k / (TX + 2), b = k - a * (TX + 2);
+        const int ci = i0 - 1 + a, cj = j0 - 1 + b;
+        double sx = 0.0, sy = 0.0;
+        if (a >= 1 && a <= TY && b <= TX)   // cells [i0, i0+TY) x [j0-1, j0+TX)
+            sx = sigma_xx(UL(ci, cj), UL(ci, cj + 1), VL(ci, cj), VL(ci + 1, cj), S.E[a][b] * S.H[a][b], dx_inv, cj,
+                          nx).res;
+        if (a <= TY && b >= 1 && b <= TX)   // cells [i0-1, i0+TY) x [j0, j0+TX)
+            sy = sigma_yy(UL(ci, cj), UL(ci, cj + 1), VL(ci, cj), VL(ci + 1, cj), S.E[a][b] * S.H[a][b], dx_inv, ci,
+                          ny).res;
+        S.sxx[a][b] = sx;
+        S.syy[a][b] = sy;
+    }
+    for (int k = tid; k < (TY + 1) * (TX + 1); k += NT) {
+        const int a = k / (TX + 1), b = k - a * (TX + 1);
+        const int vi = i0 + a, vj = j0 + b;
+        const double eH = etaH_vertex(EC(vi - 1, vj - 1), EC(vi - 1, vj), EC(vi, vj - 1), EC(vi, vj),
+                                      HC(vi - 1, vj - 1), HC(vi - 1, vj), HC(vi, vj - 1), HC(vi, vj));
+        S.sxy[a][b] = sigma_xy(UL(vi - 1, vj), UL(vi, vj), VL(vi, vj - 1), VL(vi, vj), eH, dx_inv, vi, vj, ny, nx).res;
+    }
+    __syncthreads();
+    // 4. rows of this thread's point (i, j)
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    if (i > ny || j > nx) return;
+    const int a = threadIdx.y + 1, b = threadIdx.x + 1;   // cell (i, j) in the one-ring arrays
+    auto SXX = [&](int ci, int cj) { return S.sxx[ci - i0 + 1][cj - j0 + 1]; };
+    auto SYY = [&](int ci, int cj) { return S.syy[ci - i0 + 1][cj - j0 + 1]; };
+    auto SXY = [&](int vi, int vj) { return S.sxy[vi - i0][vj - j0]; };
+    if (i < ny && j < nx) {
+        double H_c = S.H[a][b];
+        double rH = H_c / dt - (f ? f[nu + nv + (size_t)i * nx + j] : 0.0);
+        Calv jc = calving_g(H_c, Ground{S.g[a][b], S.gd[a][b]}, p);
+        rH -= jc.res;
+        double H_l = S.H[a][b - 1];
+        Flux q = flux_x(UL(i, j), H_l, H_c, j, nx);
+        rH -= q.res * dx_inv;
+        double H_r = S.H[a][b + 1];
+        q = flux_x(UL(i, j + 1), H_c, H_r, j + 1, nx);
+        rH += q.res * dx_inv;
+        double H_t = S.H[a - 1][b];
+        q = flux_y(VL(i, j), H_t, H_c, i, ny);
+        rH += q.res * dx_inv;
+        double H_b = S.H[a + 1][b];
+        q = flux_y(VL(i + 1, j), H_c, H_b, i + 1, ny);
+        rH -= q.res * dx_inv;
+        const size_t kc = (size_t)i * nx + j;
+        double m = mask[kc];
+        double th = gamma[kc];
+        double val = (1.0 - m) * rH + m * (H_c - th);
+        size_t k = nu + nv + kc;
+        r[k] = sub ? val - sub[k] : val;
+    }
+    if (i < ny) {
+        const size_t k = (size_t)i * (nx + 1) + j;
+        double ru = f ? -f[k] : -0.0;
+        ru += SXX(i, j) * dx_inv;
+        ru -= SXX(i, j - 1) * dx_inv;
+        ru += SXY(i, j) * dx_inv;
+        ru -= SXY(i + 1, j) * dx_inv;
+        {
+            TauB t = tau_b_g(UL(i, j), S.g[a][b - 1], S.g[a][b], S.gd[a][b - 1], S.gd[a][b], S.beta[a][b - 1],
+                             S.beta[a][b], p);
+            ru += t.res;
+        }
+        {
+            TauD t = tau_dx(S.H[a][b - 1], S.H[a][b], S.bed[a][b - 1], S.bed[a][b], dx_inv, j, nx);
+            ru -= t.res;
+        }
+        r[k] = sub ? ru - sub[k] : ru;
+    }
+    if (j < nx) {
+        const size_t k = nu + (size_t)i * nx + j;
+        double rv = f ? -f[k] : -0.0;
+        rv += SYY(i - 1, j) * dx_inv;
+        rv -= SYY(i, j) * dx_inv;
+        rv -= SXY(i, j) * dx_inv;
+        rv += SXY(i, j + 1) * dx_inv;
+        {
+            TauB t = tau_b_g(VL(i, j), S.g[a - 1][b], S.g[a][b], S.gd[a - 1][b], S.gd[a][b], S.beta[a - 1][b],
+                             S.beta[a][b], p);
+            rv += t.res;
+        }
+        {
+            TauD t = tau_dy(S.H[a - 1][b], S.H[a][b], S.bed[a - 1][b], S.bed[a][b], dx_inv, i, ny);
+            rv -= t.res;
+        }
+        r[k] = sub ? rv - sub[k] : rv;
+    }
+}
+
 template <class EtaFn>
 __device__ __forceinline__ CellNbhd load_nbhd(int i, int j, int ny, int nx, size_t nu, size_t nv,
                                               const double *__restrict__ u, const double *__restrict__ v,

@@ -35,6 +35,8 @@ class EnsembleStats:
     history: list = field(default_factory=list)      # relative residual after each V-cycle (tol mode)
     converged: list = field(default_factory=list)
     wall_s: float = 0.0
+    r_start: object = None                            # per-member norm at the start (tol mode)
+    converged_at_start: bool = False
 
 
 class Ensemble:
@@ -123,15 +125,18 @@ class Ensemble:
         _lib.call("glide_batch_residual", self._h, self.n_phases if phase is None else int(phase), self._stream())
 
     def forward(self, dt: float, n_vcycles: int = 3, tol: float | None = None, max_vcycles: int | None = None,
-                update_geometry: bool = True) -> EnsembleStats:
+                update_geometry: bool = True, r0=None, setup: bool = True) -> EnsembleStats:
         """One implicit time step of every member (glide/physics.py:141-209).
 
         Without ``tol`` every member runs ``n_vcycles`` V-cycles.  With ``tol``
-        members iterate until their relative residual reaches ``tol`` (at most
-        ``max_vcycles``); a converged member is disabled, so the remaining
-        launches skip its tiles."""
+        members iterate until their residual norm reaches ``tol`` times ``r0``
+        (per member; default: the norm at the start of the step) in at most
+        ``max_vcycles``; a converged member is disabled, so the remaining
+        launches skip its tiles.  ``setup=False`` keeps a forcing already set
+        by forward_setup (e.g. before seeding the velocities)."""
         st = EnsembleStats()
-        self.forward_setup(dt)
+        if setup:
+            self.forward_setup(dt)
         on = [True] * self.n
         self.set_enabled(on)
         torch.cuda.synchronize(self.device)
@@ -147,12 +152,20 @@ class Ensemble:
             st.history = [[] for _ in range(self.n)]
         else:
             n_max = max_vcycles or max(n_vcycles, 50)
-            r0 = self.residual_norms()
+            r_start = self.residual_norms()
+            r0 = r_start if r0 is None else np.asarray(r0, dtype=np.float64)
+            st.r_start = r_start
+            for m in range(self.n):
+                if r_start[m] <= tol * r0[m]:
+                    on[m] = False
+                    st.converged_at_start = True
+            if not all(on):
+                self.set_enabled(on)
             st.vcycles = [0] * self.n
-            st.latency_s = [None] * self.n
-            st.converged = [False] * self.n
+            st.latency_s = [None if on[m] else 0.0 for m in range(self.n)]
+            st.converged = [not on[m] for m in range(self.n)]
             st.history = [[] for _ in range(self.n)]
-            for it in range(n_max):
+            for it in range(n_max if any(on) else 0):
                 self.vcycle()
                 r = self.residual_norms()
                 t = time.perf_counter() - t0
