@@ -252,14 +252,11 @@ def learned_batched_bench(net, n_members=16, dt=10.0):
     """16 seeded 512^2 sheets as one ensemble: plain batched FASCD vs batched
     network inference + batched FASCD, both to 1e-8 of each member's cold
     initial residual."""
-    import ctypes
-
     import torch
 
-    from glide_amd import _lib
     from glide_amd.ensemble import Ensemble
     from glide_amd.train_warmstart import TEST_SEED0, member
-    from glide_amd.warmstart import IN_CH, VEL_SCALE, features
+    from glide_amd.warmstart import IN_CH, N_OUT, apply_prediction, features
 
     seeds = [TEST_SEED0 + 100 + k for k in range(n_members)]
     out = {"members": n_members, "size": f"{N512}x{N512}"}
@@ -277,16 +274,14 @@ def learned_batched_bench(net, n_members=16, dt=10.0):
             for k, ph in enumerate(mem):
                 features(ph, out=X[k])
             pred = net.forward(X, n_members, N512, N512)
-            st_ = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
             for k, ph in enumerate(mem):
-                g = ph.grid
-                _lib.call("glide_unet_to_faces", ctypes.c_void_p(pred[k * N512 * N512 * 2:].data_ptr()), g.ny, g.nx,
-                          VEL_SCALE, g.u.data_ptr(), g.v.data_ptr(), st_)
+                apply_prediction(ph, pred[k * N512 * N512 * N_OUT:(k + 1) * N512 * N512 * N_OUT])
             rw = ens.residual_norms()
             for k, ph in enumerate(mem):       # per-member fallback
                 if not rw[k] <= r0[k]:
                     ph.grid.u.zero_()
                     ph.grid.v.zero_()
+                    ph.grid.H.copy_(ph.grid.H_prev)
         st = ens.forward(dt, tol=TOL, max_vcycles=300, update_geometry=False, r0=r0, setup=False)
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) * 1e3

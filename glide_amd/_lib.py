@@ -102,17 +102,18 @@ SIGNATURES = {
     "glide_unet_up2_bwd": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p]),
     "glide_unet_bias_chunks": (ctypes.c_long, [ctypes.c_long]),
     "glide_unet_bias_grad": (c_int, [c_void_p, ctypes.c_long, c_int, c_void_p, c_void_p, c_void_p]),
-    "glide_unet_head": (c_int, [c_void_p, ctypes.c_long, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
-    "glide_unet_head_bwd": (c_int, [c_void_p, c_void_p, ctypes.c_long, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
-                                    c_void_p]),
+    "glide_unet_head": (c_int, [c_void_p, ctypes.c_long, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "glide_unet_head_bwd": (c_int, [c_void_p, c_void_p, ctypes.c_long, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p]),
     "glide_unet_mse_blocks": (ctypes.c_long, [ctypes.c_long]),
     "glide_unet_mse": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_long, ctypes.c_float, c_void_p, c_void_p,
                                c_void_p]),
     "glide_unet_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_long, ctypes.c_float,
                                 ctypes.c_float, ctypes.c_float, ctypes.c_float, c_int, ctypes.c_float, c_void_p]),
     "glide_unet_features": (c_int, [DP, DP, DP, DP, c_int, c_int, c_double, c_void_p, c_void_p]),
-    "glide_unet_targets": (c_int, [DP, DP, c_int, c_int, c_double, c_void_p, c_void_p]),
-    "glide_unet_to_faces": (c_int, [c_void_p, c_int, c_int, c_double, DP, DP, c_void_p]),
+    "glide_unet_targets": (c_int, [DP, DP, DP, DP, c_int, c_int, c_double, c_double, c_void_p, c_void_p]),
+    "glide_unet_apply_prediction": (c_int, [c_void_p, c_int, c_int, c_double, c_double, DP, DP, DP, DP, DP,
+                                            c_void_p]),
     "glide_last_error": (c_char_p, []),
     "glide_version": (c_char_p, []),
 }

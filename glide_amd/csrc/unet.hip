@@ -359,47 +359,50 @@ __global__ void k_bias_grad(const bf16_t *__restrict__ dz, long P, int C, long c
     part[(long)blockIdx.y * C + c] = s;
 }
 
-// 1x1 output head: out[p][o] = b[o] + sum_c Wh[o][c] x[p][c]  (FP32, 2 outputs)
-__global__ void k_head(const bf16_t *__restrict__ x, long P, int C, const float *__restrict__ Wh,
+// 1x1 output head with NO <= 4 outputs:
+// out[p][o] = b[o] + sum_c Wh[o][c] x[p][c]  (FP32)
+__global__ void k_head(const bf16_t *__restrict__ x, long P, int C, int NO, const float *__restrict__ Wh,
                        const float *__restrict__ bh, float *__restrict__ out)
 {
     long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
-    float s0 = bh[0], s1 = bh[1];
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int o = 0; o < NO; o++) s[o] = bh[o];
     for (int c = 0; c < C; c++) {
         float v = bf2f(x[p * C + c]);
-        s0 += Wh[c] * v;
-        s1 += Wh[C + c] * v;
+        for (int o = 0; o < NO; o++) s[o] += Wh[o * C + c] * v;
     }
-    out[2 * p] = s0;
-    out[2 * p + 1] = s1;
+    for (int o = 0; o < NO; o++) out[NO * p + o] = s[o];
 }
 // dx[p][c] = sum_o dout[p][o] Wh[o][c]
-__global__ void k_head_bwd_x(const float *__restrict__ dout, long P, int C, const float *__restrict__ Wh,
+__global__ void k_head_bwd_x(const float *__restrict__ dout, long P, int C, int NO, const float *__restrict__ Wh,
                              float *__restrict__ dx)
 {
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P * C) return;
     const long p = k / C;
     const int c = (int)(k % C);
-    dx[k] = dout[2 * p] * Wh[c] + dout[2 * p + 1] * Wh[C + c];
+    float s = dout[NO * p] * Wh[c];
+    for (int o = 1; o < NO; o++) s += dout[NO * p + o] * Wh[o * C + c];
+    dx[k] = s;
 }
 // partials over pixel chunks of dWh[o][c] = sum_p dout[p][o] x[p][c] and dbh[o] = sum_p dout[p][o]
-__global__ void k_head_bwd_w(const float *__restrict__ dout, const bf16_t *__restrict__ x, long P, int C, long chunk,
-                             float *__restrict__ part)
+__global__ void k_head_bwd_w(const float *__restrict__ dout, const bf16_t *__restrict__ x, long P, int C, int NO,
+                             long chunk, float *__restrict__ part)
 {
-    const int k = threadIdx.x;   // 0 .. 2C+1
-    if (k >= 2 * C + 2) return;
+    const int k = threadIdx.x;   // 0 .. NO*C + NO - 1
+    const int nk = NO * C + NO;
+    if (k >= nk) return;
     const long lo = (long)blockIdx.x * chunk, hi = lo + chunk < P ? lo + chunk : P;
     float s = 0.0f;
-    if (k < 2 * C) {
+    if (k < NO * C) {
         const int o = k / C, c = k % C;
-        for (long p = lo; p < hi; p++) s += dout[2 * p + o] * bf2f(x[p * C + c]);
+        for (long p = lo; p < hi; p++) s += dout[NO * p + o] * bf2f(x[p * C + c]);
     } else {
-        const int o = k - 2 * C;
-        for (long p = lo; p < hi; p++) s += dout[2 * p + o];
+        const int o = k - NO * C;
+        for (long p = lo; p < hi; p++) s += dout[NO * p + o];
     }
-    part[(long)blockIdx.x * (2 * C + 2) + k] = s;
+    part[(long)blockIdx.x * nk + k] = s;
 }
 
 // masked mean-squared error: loss partials and d(loss)/d(pred) = 2 m (pred - target) / count
@@ -442,9 +445,13 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float
 // ---------------------------------------------------------------------------
 // solver <-> network
 // ---------------------------------------------------------------------------
-// Network input of one member, NHWC with 32 channels (5 used):
-//   0: H / 1000, 1: bed / 1000, 2: 100 |grad s| (central differences, one-sided
-//   at the edges), 3: log(beta) + 2, 4: B / 24 - 1; channels 5..31 are 0.
+// Network input of one member, NHWC with 32 channels (7 used):
+//   0: H / 1000, 1: tanh((bed + 0.917 H) / 200) (flotation margin: the way
+//   the bed enters the momentum balance besides the surface), 2: 100 |grad s|,
+//   3: log(beta) + 2, 4: B / 24 - 1, 5, 6: 100 ds/dx, 100 ds/dy (central
+//   differences, one-sided at the edges); channels 7..31 are 0.  Every
+//   channel is invariant to a shift of the domain, so a network trained on
+//   small domains applies to large ones.
 // s = max(bed, -0.917 H) + H (the reference's surface, glide/physics.py:223-229).
 __device__ __forceinline__ double surf(const double *H, const double *bed, long k)
 {
@@ -460,37 +467,46 @@ __global__ void k_features(const double *__restrict__ H, const double *__restric
     const int it = i > 0 ? i - 1 : i, ib = i < ny - 1 ? i + 1 : i;
     const double sx = (surf(H, bed, (long)i * nx + jr) - surf(H, bed, (long)i * nx + jl)) / ((jr - jl) * dx);
     const double sy = (surf(H, bed, (long)ib * nx + j) - surf(H, bed, (long)it * nx + j)) / ((ib - it) * dx);
-    float f[5] = {(float)(H[k] / 1000.0), (float)(bed[k] / 1000.0), (float)(100.0 * sqrt(sx * sx + sy * sy)),
-                  (float)(log(beta[k]) + 2.0), (float)(B[k] / 24.0 - 1.0)};
+    float f[7] = {(float)(H[k] / 1000.0), (float)tanh((bed[k] + 0.917 * H[k]) / 200.0),
+                  (float)(100.0 * sqrt(sx * sx + sy * sy)), (float)(log(beta[k]) + 2.0), (float)(B[k] / 24.0 - 1.0),
+                  (float)(100.0 * sx), (float)(100.0 * sy)};
     bf16_t *o = x + k * 32;
-    for (int c = 0; c < 32; c++) o[c] = c < 5 ? f2bf(f[c]) : (bf16_t)0;
+    for (int c = 0; c < 32; c++) o[c] = c < 7 ? f2bf(f[c]) : (bf16_t)0;
 }
-// cell-centred velocity targets (FP32, scaled): (u_l + u_r) / 2, (v_t + v_b) / 2
-__global__ void k_targets(const double *__restrict__ u, const double *__restrict__ v, int ny, int nx, double scale,
+// targets (FP32, 3 per cell): cell-centred (u_l + u_r)/2 and (v_t + v_b)/2
+// times scale, and the thickness change (H - H_prev) times hscale
+__global__ void k_targets(const double *__restrict__ u, const double *__restrict__ v, const double *__restrict__ H,
+                          const double *__restrict__ H_prev, int ny, int nx, double scale, double hscale,
                           float *__restrict__ tgt)
 {
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)ny * nx) return;
     const int i = (int)(k / nx), j = (int)(k % nx);
-    tgt[2 * k] = (float)(0.5 * (u[(long)i * (nx + 1) + j] + u[(long)i * (nx + 1) + j + 1]) * scale);
-    tgt[2 * k + 1] = (float)(0.5 * (v[(long)i * nx + j] + v[(long)(i + 1) * nx + j]) * scale);
+    tgt[3 * k] = (float)(0.5 * (u[(long)i * (nx + 1) + j] + u[(long)i * (nx + 1) + j + 1]) * scale);
+    tgt[3 * k + 1] = (float)(0.5 * (v[(long)i * nx + j] + v[(long)(i + 1) * nx + j]) * scale);
+    tgt[3 * k + 2] = (float)((H[k] - H_prev[k]) * hscale);
 }
-// faces from cell-centred predictions: interior faces average their two
-// cells, boundary faces take their one cell; values divided by scale
-__global__ void k_to_faces(const float *__restrict__ pred, int ny, int nx, double scale, double *__restrict__ u,
-                           double *__restrict__ v)
+// the solver state from a prediction (3 per cell): faces from the cell-centred
+// velocities / scale (interior faces average their two cells, boundary faces
+// take their one cell); H = max(H_prev + dH / hscale, gamma)
+__global__ void k_apply_prediction(const float *__restrict__ pred, int ny, int nx, double scale, double hscale,
+                                   double *__restrict__ u, double *__restrict__ v, double *__restrict__ H,
+                                   const double *__restrict__ H_prev, const double *__restrict__ gamma)
 {
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long nu = (long)ny * (nx + 1), nv = (long)(ny + 1) * nx;
+    const long nu = (long)ny * (nx + 1), nv = (long)(ny + 1) * nx, nc = (long)ny * nx;
     if (k < nu) {
         const int i = (int)(k / (nx + 1)), j = (int)(k % (nx + 1));
         const int jl = j > 0 ? j - 1 : 0, jr = j < nx ? j : nx - 1;
-        u[k] = 0.5 * ((double)pred[2 * ((long)i * nx + jl)] + (double)pred[2 * ((long)i * nx + jr)]) / scale;
+        u[k] = 0.5 * ((double)pred[3 * ((long)i * nx + jl)] + (double)pred[3 * ((long)i * nx + jr)]) / scale;
     } else if (k < nu + nv) {
         const long kk = k - nu;
         const int i = (int)(kk / nx), j = (int)(kk % nx);
         const int it = i > 0 ? i - 1 : 0, ib = i < ny ? i : ny - 1;
-        v[kk] = 0.5 * ((double)pred[2 * ((long)it * nx + j) + 1] + (double)pred[2 * ((long)ib * nx + j) + 1]) / scale;
+        v[kk] = 0.5 * ((double)pred[3 * ((long)it * nx + j) + 1] + (double)pred[3 * ((long)ib * nx + j) + 1]) / scale;
+    } else if (k < nu + nv + nc) {
+        const long kk = k - nu - nv;
+        H[kk] = fmax(H_prev[kk] + (double)pred[3 * kk + 2] / hscale, gamma[kk]);
     }
 }
 
@@ -633,25 +649,26 @@ int glide_unet_bias_grad(const uint16_t *dz, long P, int C, float *partial, floa
     return ucheck("bias_grad");
 }
 
-int glide_unet_head(const uint16_t *x, long P, int C, const float *wh, const float *bh, float *out, void *stream)
+int glide_unet_head(const uint16_t *x, long P, int C, int NO, const float *wh, const float *bh, float *out,
+                    void *stream)
 {
     if (!x || !wh || !bh || !out) return ufail(GLIDE_E_NULL, "head: NULL argument");
-    if (P < 1 || C < 1) return ufail(GLIDE_E_SIZE, "head: bad size");
-    hipLaunchKernelGGL(k_head, dim3(nblk(P)), dim3(256), 0, SS(stream), x, P, C, wh, bh, out);
+    if (P < 1 || C < 1 || NO < 1 || NO > 4) return ufail(GLIDE_E_SIZE, "head: bad size (1 <= outputs <= 4)");
+    hipLaunchKernelGGL(k_head, dim3(nblk(P)), dim3(256), 0, SS(stream), x, P, C, NO, wh, bh, out);
     return ucheck("head");
 }
 
-int glide_unet_head_bwd(const float *dout, const uint16_t *x, long P, int C, const float *wh, float *dx,
+int glide_unet_head_bwd(const float *dout, const uint16_t *x, long P, int C, int NO, const float *wh, float *dx,
                         float *partial, float *dwh_dbh, void *stream)
 {
     if (!dout || !x || !wh || !dx || !partial || !dwh_dbh) return ufail(GLIDE_E_NULL, "head_bwd: NULL argument");
-    if (P < 1 || C < 1 || 2 * C + 2 > 1024) return ufail(GLIDE_E_SIZE, "head_bwd: bad size");
-    hipLaunchKernelGGL(k_head_bwd_x, dim3(nblk(P * C)), dim3(256), 0, SS(stream), dout, P, C, wh, dx);
+    const int nk = NO * C + NO;
+    if (P < 1 || C < 1 || NO < 1 || NO > 4 || nk > 1024) return ufail(GLIDE_E_SIZE, "head_bwd: bad size");
+    hipLaunchKernelGGL(k_head_bwd_x, dim3(nblk(P * C)), dim3(256), 0, SS(stream), dout, P, C, NO, wh, dx);
     const long nz = glide_unet_bias_chunks(P);
-    hipLaunchKernelGGL(k_head_bwd_w, dim3((unsigned)nz), dim3(2 * C + 2 > 64 ? ((2 * C + 2 + 63) / 64) * 64 : 64), 0,
-                       SS(stream), dout, x, P, C, 4096L, partial);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk(2 * C + 2)), dim3(256), 0, SS(stream), partial, (long)(2 * C + 2),
-                       (int)nz, dwh_dbh);
+    hipLaunchKernelGGL(k_head_bwd_w, dim3((unsigned)nz), dim3(((nk + 63) / 64) * 64), 0, SS(stream), dout, x, P, C,
+                       NO, 4096L, partial);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk(nk)), dim3(256), 0, SS(stream), partial, (long)nk, (int)nz, dwh_dbh);
     return ucheck("head_bwd");
 }
 
@@ -688,21 +705,26 @@ int glide_unet_features(const double *H, const double *bed, const double *beta, 
     return ucheck("features");
 }
 
-int glide_unet_targets(const double *u, const double *v, int ny, int nx, double scale, float *tgt, void *stream)
+int glide_unet_targets(const double *u, const double *v, const double *H, const double *H_prev, int ny, int nx,
+                       double scale, double hscale, float *tgt, void *stream)
 {
-    if (!u || !v || !tgt) return ufail(GLIDE_E_NULL, "targets: NULL argument");
+    if (!u || !v || !H || !H_prev || !tgt) return ufail(GLIDE_E_NULL, "targets: NULL argument");
     if (ny < 1 || nx < 1) return ufail(GLIDE_E_SIZE, "targets: bad grid");
-    hipLaunchKernelGGL(k_targets, dim3(nblk((long)ny * nx)), dim3(256), 0, SS(stream), u, v, ny, nx, scale, tgt);
+    hipLaunchKernelGGL(k_targets, dim3(nblk((long)ny * nx)), dim3(256), 0, SS(stream), u, v, H, H_prev, ny, nx, scale,
+                       hscale, tgt);
     return ucheck("targets");
 }
 
-int glide_unet_to_faces(const float *pred, int ny, int nx, double scale, double *u, double *v, void *stream)
+int glide_unet_apply_prediction(const float *pred, int ny, int nx, double scale, double hscale, double *u, double *v,
+                                double *H, const double *H_prev, const double *gamma, void *stream)
 {
-    if (!pred || !u || !v) return ufail(GLIDE_E_NULL, "to_faces: NULL argument");
-    if (ny < 1 || nx < 1 || !(scale > 0.0)) return ufail(GLIDE_E_SIZE, "to_faces: bad grid or scale");
-    const long n = (long)ny * (nx + 1) + (long)(ny + 1) * nx;
-    hipLaunchKernelGGL(k_to_faces, dim3(nblk(n)), dim3(256), 0, SS(stream), pred, ny, nx, scale, u, v);
-    return ucheck("to_faces");
+    if (!pred || !u || !v || !H || !H_prev || !gamma) return ufail(GLIDE_E_NULL, "apply_prediction: NULL argument");
+    if (ny < 1 || nx < 1 || !(scale > 0.0) || !(hscale > 0.0))
+        return ufail(GLIDE_E_SIZE, "apply_prediction: bad grid or scale");
+    const long n = (long)ny * (nx + 1) + (long)(ny + 1) * nx + (long)ny * nx;
+    hipLaunchKernelGGL(k_apply_prediction, dim3(nblk(n)), dim3(256), 0, SS(stream), pred, ny, nx, scale, hscale, u, v,
+                       H, H_prev, gamma);
+    return ucheck("apply_prediction");
 }
 
 }  // extern "C"

@@ -6,8 +6,9 @@
    10000...), ``size`` x ``size`` cells, dx alternating 1000 / 1500 m, each
    solved by batched FASCD (Ensemble, ``batch`` members per plan) from a cold
    start (u = v = 0) to ``tol`` relative residual; the network input is the
-   member's (H, bed, |grad s|, log beta, B) and the target its converged,
-   cell-centred (u, v).  Everything stays in HBM.
+   member's cell fields (glide_unet_features) and the target its converged,
+   cell-centred (u, v) and thickness change H - H_prev; solves that do not reach ``tol`` in ``max_vcycles``
+   are left out.  Everything stays in HBM.
 2. Training: Adam on the GPU (BF16 MFMA forward and backward kernels, FP32
    master weights), cosine learning-rate decay, 5% of the samples held out.
 3. Evaluation: on test seeds (20000...) the number of V-cycles a forward step
@@ -27,7 +28,7 @@ import torch
 from . import synthetic as S
 from .ensemble import Ensemble
 from .physics import IcePhysics
-from .warmstart import IN_CH, VEL_SCALE, UNet, WarmStart, features, targets
+from .warmstart import DH_SCALE, IN_CH, N_OUT, VEL_SCALE, UNet, WarmStart, features, targets
 
 TRAIN_SEED0 = 10000
 TEST_SEED0 = 20000
@@ -47,11 +48,12 @@ def member(seed: int, size: int, dx: float | None = None, n_levels: int | None =
 
 
 def generate(n: int, size: int, seed0: int = TRAIN_SEED0, dt: float = 10.0, tol: float = 1e-8, batch: int = 64,
-             max_vcycles: int = 80, device=None, log=print):
-    """Solve n seeded sheets; returns (X int16 [n, size*size*32], Y fp32 [n, size*size*2], stats)."""
+             max_vcycles: int = 80, device=None, log=print, drop_unconverged: bool = True):
+    """Solve n seeded sheets; returns (X int16 [m, size*size*32], Y fp32 [m, size*size*3], stats),
+    keeping only the m solves that reached ``tol`` unless ``drop_unconverged`` is False."""
     dev = torch.device(device if device is not None else "cuda")
     X = torch.empty((n, size * size * IN_CH), dtype=torch.int16, device=dev)
-    Y = torch.empty((n, size * size * 2), dtype=torch.float32, device=dev)
+    Y = torch.empty((n, size * size * N_OUT), dtype=torch.float32, device=dev)
     vcycles, conv = [], []
     t0 = time.perf_counter()
     for c0 in range(0, n, batch):
@@ -69,8 +71,13 @@ def generate(n: int, size: int, seed0: int = TRAIN_SEED0, dt: float = 10.0, tol:
         log(f"  solved {min(n, c0 + batch)}/{n} ({time.perf_counter() - t0:.1f} s; "
             f"V-cycles {min(st.vcycles)}-{max(st.vcycles)})")
     torch.cuda.synchronize(dev)
-    return X, Y, {"solve_s": time.perf_counter() - t0, "vcycles_mean": float(np.mean(vcycles)),
-                  "vcycles_max": int(np.max(vcycles)), "converged": int(np.sum(conv)), "n": n}
+    stats = {"solve_s": time.perf_counter() - t0, "vcycles_mean": float(np.mean(vcycles)),
+             "vcycles_max": int(np.max(vcycles)), "converged": int(np.sum(conv)), "n": n}
+    if drop_unconverged and not all(conv):
+        keep = torch.as_tensor(np.flatnonzero(np.asarray(conv, dtype=bool)), device=dev)
+        X, Y = X.index_select(0, keep), Y.index_select(0, keep)
+        stats["kept"] = int(keep.numel())
+    return X, Y, stats
 
 
 def train(net: UNet, X: torch.Tensor, Y: torch.Tensor, size: int, epochs: int = 12, batch: int = 16,
@@ -132,7 +139,7 @@ def evaluate(net: UNet, seeds, size: int, dx: float | None = None, n_levels: int
         cold = ws.solve(ph, dt, tol=tol, max_vcycles=max_vcycles, use_net=False)
         torch.cuda.synchronize()
         t_cold = time.perf_counter() - t0
-        u_ref, v_ref = ph.grid.u.clone(), ph.grid.v.clone()
+        u_ref, v_ref, H_ref = ph.grid.u.clone(), ph.grid.v.clone(), ph.grid.H.clone()
         ph.grid.u.zero_()
         ph.grid.v.zero_()
         torch.cuda.synchronize()
@@ -144,9 +151,12 @@ def evaluate(net: UNet, seeds, size: int, dx: float | None = None, n_levels: int
         # prediction error (before any V-cycle) vs the converged cold solve
         ph.grid.u.zero_()
         ph.grid.v.zero_()
+        ph.grid.H.copy_(ph.grid.H_prev)
         ws.predict_into(ph)
         num = torch.sqrt(((ph.grid.u - u_ref) ** 2).sum() + ((ph.grid.v - v_ref) ** 2).sum())
         den = torch.sqrt((u_ref ** 2).sum() + (v_ref ** 2).sum())
+        dH_ref = H_ref - ph.grid.H_prev
+        dH_err = float(torch.linalg.norm(ph.grid.H - H_ref) / torch.linalg.norm(dH_ref))
         diff = torch.sqrt(((u_w - u_ref) ** 2).sum() + ((v_w - v_ref) ** 2).sum())
         out.append({"seed": seed, "size": size, "dx": ph.dx, "levels": ph.n_levels,
                     "vcycles_cold": cold["vcycles"], "vcycles_warm": warm["vcycles"],
@@ -154,7 +164,7 @@ def evaluate(net: UNet, seeds, size: int, dx: float | None = None, n_levels: int
                     "ms_cold": t_cold * 1e3, "ms_warm": t_warm * 1e3,
                     "r0_warm_over_cold": (warm["r0_warm"] / cold["r0_cold"]) if warm["r0_warm"] is not None else None,
                     "fallback": warm["fallback"],
-                    "prediction_rel_error": float(num / den),
+                    "prediction_rel_error": float(num / den), "thickness_change_rel_error": dH_err,
                     "solution_rel_diff_warm_vs_cold": float(diff / den)})
         del ph
         torch.cuda.empty_cache()
@@ -177,7 +187,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n-train", type=int, default=2000)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--tol", type=float, default=1e-8)
@@ -198,7 +208,7 @@ def main():
     tr = train(net, X, Y, args.size, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
     del X, Y
     torch.cuda.empty_cache()
-    net.save(args.out, vel_scale=VEL_SCALE, train_size=args.size, n_train=args.n_train, epochs=args.epochs,
+    net.save(args.out, vel_scale=VEL_SCALE, dh_scale=DH_SCALE, train_size=args.size, n_train=args.n_train, epochs=args.epochs,
              final_val_loss=tr["history"][-1]["val_loss"])
     report = {"generate": gen, "train": tr}
     for size in args.test_size:

@@ -1,10 +1,11 @@
 """Learned warm start of the FASCD forward solve.
 
-A 4-level convolutional U-Net (32/64/96/96 channels, 516,898 parameters)
-maps the cell fields (H, bed, |grad s|, log beta, B) of a domain to
-cell-centred (u, v); the prediction, mapped to the MAC faces, replaces the
-cold initial velocities of a forward step.  Plain FASCD remains the solver and
-the judge: the step iterates V-cycles until the reference's residual norm
+A 4-level convolutional U-Net (32/64/96/96 channels, 516,931 parameters)
+maps the cell fields of a domain (H, the bed as flotation margin, |grad s|,
+log beta, B and the signed surface slopes) to the cell-centred (u, v) and
+the thickness change H - H_prev of a forward step; the prediction, mapped to
+the MAC faces and to H = max(H_prev + dH, gamma), replaces the cold initial
+state of the step.  Plain FASCD remains the solver and the judge: the step iterates V-cycles until the reference's residual norm
 (glide/physics.py:176-203) drops below ``tol`` times the COLD start's initial
 norm, and if the predicted state's initial residual exceeds the cold one the
 prediction is discarded (fallback to the cold start).
@@ -40,6 +41,8 @@ UPS = {"u3": "b", "u2": "d3", "u1": "d2"}
 IN_CH = 32
 HEAD_IN = "d1"
 VEL_SCALE = 0.01          # network units: m/a x 0.01
+DH_SCALE = 0.01           # network units of the step's thickness change: m x 0.01
+N_OUT = 3                 # outputs per cell: u, v, H - H_prev
 DOWNSAMPLE = 8            # domain sides must be multiples of this
 
 
@@ -60,7 +63,7 @@ SHAPES = {name: (co, sum(CH[i] for i in ins)) for name, ins, co in LAYERS}
 
 
 def n_params() -> int:
-    return sum(9 * co * ci + co for co, ci in SHAPES.values()) + 2 * CH[HEAD_IN] + 2
+    return sum(9 * co * ci + co for co, ci in SHAPES.values()) + N_OUT * CH[HEAD_IN] + N_OUT
 
 
 def _p(t):
@@ -81,9 +84,9 @@ class UNet:
             self.params[name + ".W"] = torch.tensor(w, dtype=torch.float32, device=self.device)
             self.params[name + ".b"] = torch.zeros(co, dtype=torch.float32, device=self.device)
         c = CH[HEAD_IN]
-        self.params["head.W"] = torch.tensor(rng.standard_normal((2, c)) * math.sqrt(1.0 / c), dtype=torch.float32,
-                                             device=self.device)
-        self.params["head.b"] = torch.zeros(2, dtype=torch.float32, device=self.device)
+        self.params["head.W"] = torch.tensor(rng.standard_normal((N_OUT, c)) * math.sqrt(1.0 / c),
+                                             dtype=torch.float32, device=self.device)
+        self.params["head.b"] = torch.zeros(N_OUT, dtype=torch.float32, device=self.device)
         self.grads = {k: torch.zeros_like(v) for k, v in self.params.items()}
         self.m = {k: torch.zeros_like(v) for k, v in self.params.items()}
         self.v = {k: torch.zeros_like(v) for k, v in self.params.items()}
@@ -117,7 +120,7 @@ class UNet:
             h, w = dims[LEVEL[name]]
             acts[name] = torch.empty(N * h * w * CH[name], dtype=torch.int16, device=self.device)
         b = {"acts": acts, "dims": dims,
-             "pred": torch.empty(N * H * W * 2, dtype=torch.float32, device=self.device)}
+             "pred": torch.empty(N * H * W * N_OUT, dtype=torch.float32, device=self.device)}
         if train:
             g = {}
             for name in CH:
@@ -132,11 +135,12 @@ class UNet:
                 h, w = dims[LEVEL[name]]
                 nz = _lib.fn("glide_unet_wgrad_chunks")(N, h, w)
                 part = max(part, nz * 9 * co * SHAPES[name][1], _lib.fn("glide_unet_bias_chunks")(N * h * w) * co)
-            part = max(part, _lib.fn("glide_unet_bias_chunks")(N * H * W) * (2 * CH[HEAD_IN] + 2),
-                       _lib.fn("glide_unet_mse_blocks")(N * H * W * 2))
+            nk = N_OUT * CH[HEAD_IN] + N_OUT
+            part = max(part, _lib.fn("glide_unet_bias_chunks")(N * H * W) * nk,
+                       _lib.fn("glide_unet_mse_blocks")(N * H * W * N_OUT))
             b["partial"] = torch.empty(part, dtype=torch.float32, device=self.device)
-            b["dpred"] = torch.empty(N * H * W * 2, dtype=torch.float32, device=self.device)
-            b["head"] = torch.empty(2 * CH[HEAD_IN] + 2, dtype=torch.float32, device=self.device)
+            b["dpred"] = torch.empty(N * H * W * N_OUT, dtype=torch.float32, device=self.device)
+            b["head"] = torch.empty(nk, dtype=torch.float32, device=self.device)
         self._bufs[key] = b
         return b
 
@@ -145,7 +149,7 @@ class UNet:
 
     # ------------------------------------------------------------ forward / backward
     def forward(self, x0: torch.Tensor, N: int, H: int, W: int, train: bool = False) -> torch.Tensor:
-        """x0: N x H x W x 32 BF16 bits (int16 tensor).  Returns N x H x W x 2 FP32."""
+        """x0: N x H x W x 32 BF16 bits (int16 tensor).  Returns N x H x W x 3 FP32."""
         b = self._buffers(N, H, W, train)
         acts, dims, st = b["acts"], b["dims"], self._stream()
         acts["x0"] = x0
@@ -163,7 +167,7 @@ class UNet:
             xa, xb = acts[ins[0]], (acts[ins[1]] if len(ins) > 1 else None)
             _lib.call("glide_unet_conv3x3", _p(xa), CH[ins[0]], _p(xb), CH[ins[1]] if xb is not None else 0, N, h, w,
                       _p(self.wb[name]), co, _p(self.params[name + ".b"]), 1, _p(acts[name]), st)
-        _lib.call("glide_unet_head", _p(acts[HEAD_IN]), N * H * W, CH[HEAD_IN], _p(self.params["head.W"]),
+        _lib.call("glide_unet_head", _p(acts[HEAD_IN]), N * H * W, CH[HEAD_IN], N_OUT, _p(self.params["head.W"]),
                   _p(self.params["head.b"]), _p(b["pred"]), st)
         return b["pred"]
 
@@ -171,7 +175,7 @@ class UNet:
         """MSE of the last forward's prediction; d(loss)/d(pred) into the
         training buffers.  Returns the loss as a 0-d device tensor."""
         b = self._bufs[(N, H, W, True)]
-        n = N * H * W * 2
+        n = N * H * W * N_OUT
         nb = _lib.fn("glide_unet_mse_blocks")(n)
         _lib.call("glide_unet_mse", _p(b["pred"]), _p(target), None, n, ctypes.c_float(1.0 / n), _p(b["dpred"]),
                   _p(b["partial"]), self._stream())
@@ -184,10 +188,10 @@ class UNet:
         dpred = b["dpred"] if dpred is None else dpred
         P0 = N * H * W
         C = CH[HEAD_IN]
-        _lib.call("glide_unet_head_bwd", _p(dpred), _p(acts[HEAD_IN]), P0, C, _p(self.params["head.W"]),
+        _lib.call("glide_unet_head_bwd", _p(dpred), _p(acts[HEAD_IN]), P0, C, N_OUT, _p(self.params["head.W"]),
                   _p(g[HEAD_IN]), _p(b["partial"]), _p(b["head"]), st)
-        self.grads["head.W"].copy_(b["head"][:2 * C].view(2, C))
-        self.grads["head.b"].copy_(b["head"][2 * C:])
+        self.grads["head.W"].copy_(b["head"][:N_OUT * C].view(N_OUT, C))
+        self.grads["head.b"].copy_(b["head"][N_OUT * C:])
         dz = b["dz"]
         for name, ins, co in reversed(LAYERS):
             h, w = dims[LEVEL[name]]
@@ -257,30 +261,46 @@ def features(physics, out: torch.Tensor | None = None) -> torch.Tensor:
     return x
 
 
-def targets(physics, out: torch.Tensor | None = None, scale: float = VEL_SCALE) -> torch.Tensor:
+def targets(physics, out: torch.Tensor | None = None, scale: float = VEL_SCALE,
+            hscale: float = DH_SCALE) -> torch.Tensor:
+    """Training targets of a solved step: cell-centred (u, v) x scale and
+    (H - H_prev) x hscale, 3 FP32 values per cell."""
     g = physics.grid
-    t = out if out is not None else torch.empty(g.ny * g.nx * 2, dtype=torch.float32, device=g.device)
+    t = out if out is not None else torch.empty(g.ny * g.nx * N_OUT, dtype=torch.float32, device=g.device)
     st = ctypes.c_void_p(torch.cuda.current_stream(g.device).cuda_stream)
-    _lib.call("glide_unet_targets", g.u.data_ptr(), g.v.data_ptr(), g.ny, g.nx, float(scale), _p(t), st)
+    _lib.call("glide_unet_targets", g.u.data_ptr(), g.v.data_ptr(), g.H.data_ptr(), g.H_prev.data_ptr(), g.ny, g.nx,
+              float(scale), float(hscale), _p(t), st)
     return t
 
 
-class WarmStart:
-    """Seeds IcePhysics forward steps with the network's (u, v)."""
+def apply_prediction(physics, pred: torch.Tensor, scale: float = VEL_SCALE, hscale: float = DH_SCALE):
+    """Write a prediction (ny x nx x 3) into the finest level: u, v faces and
+    H = max(H_prev + dH, gamma)."""
+    g = physics.grid
+    st = ctypes.c_void_p(torch.cuda.current_stream(g.device).cuda_stream)
+    _lib.call("glide_unet_apply_prediction", _p(pred), g.ny, g.nx, float(scale), float(hscale), g.u.data_ptr(),
+              g.v.data_ptr(), g.H.data_ptr(), g.H_prev.data_ptr(), g.gamma.data_ptr(), st)
 
-    def __init__(self, net: UNet, scale: float = VEL_SCALE):
-        self.net, self.scale = net, float(scale)
+
+class WarmStart:
+    """Seeds IcePhysics forward steps with the network's (u, v) and, with
+    ``seed_thickness``, its H = max(H_prev + dH, gamma) (else H = H_prev)."""
+
+    def __init__(self, net: UNet, scale: float = VEL_SCALE, hscale: float = DH_SCALE, seed_thickness: bool = True):
+        self.net, self.scale, self.hscale = net, float(scale), float(hscale)
+        self.seed_thickness = bool(seed_thickness)
 
     def predict_into(self, physics):
         g = physics.grid
         x = features(physics)
         pred = self.net.forward(x, 1, g.ny, g.nx)
-        st = ctypes.c_void_p(torch.cuda.current_stream(g.device).cuda_stream)
-        _lib.call("glide_unet_to_faces", _p(pred), g.ny, g.nx, self.scale, g.u.data_ptr(), g.v.data_ptr(), st)
+        apply_prediction(physics, pred, self.scale, self.hscale)
+        if not self.seed_thickness:
+            g.H.copy_(g.H_prev)
 
     def solve(self, physics, dt: float, tol: float = 1e-8, max_vcycles: int = 200, use_net: bool = True) -> dict:
         """One forward step (H starts at H_prev) to ||r|| <= tol * r0_cold.
-        The cold start is the state's current (u, v)."""
+        The cold start is the state's current (u, v) with H = H_prev."""
         g = physics.grid
         for lv in physics.grids:
             lv.dt = float(dt)
@@ -296,6 +316,7 @@ class WarmStart:
             if not (r_warm <= r_cold):
                 g.u.copy_(u0)
                 g.v.copy_(v0)
+                g.H.copy_(g.H_prev)
                 out["fallback"] = True
             else:
                 out["used_net"] = True

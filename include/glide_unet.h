@@ -69,13 +69,14 @@ int glide_unet_up2_bwd(const float *dy, int N, int H, int W, int C, float *dx, v
 long glide_unet_bias_chunks(long P);
 int glide_unet_bias_grad(const uint16_t *dz, long P, int C, float *partial, float *db, void *stream);
 
-/* 1x1 output head (2 outputs, FP32): out[p][o] = bh[o] + sum_c wh[o][c] x[p][c];
- * backward: dx (FP32, P x C) and [dwh (2 x C) | dbh (2)] through
- * glide_unet_bias_chunks(P) x (2C+2) partials. */
-int glide_unet_head(const uint16_t *x, long P, int C, const float *wh, const float *bh, float *out,
+/* 1x1 output head with NO (1..4) outputs, FP32:
+ * out[p][o] = bh[o] + sum_c wh[o][c] x[p][c]; backward: dx (FP32, P x C) and
+ * [dwh (NO x C) | dbh (NO)] through glide_unet_bias_chunks(P) x (NO C + NO)
+ * partials. */
+int glide_unet_head(const uint16_t *x, long P, int C, int NO, const float *wh, const float *bh, float *out,
                     void *stream);
-int glide_unet_head_bwd(const float *dout, const uint16_t *x, long P, int C, const float *wh, float *dx,
-                        float *partial, float *dwh_dbh, void *stream);
+int glide_unet_head_bwd(const float *dout, const uint16_t *x, long P, int C, int NO, const float *wh,
+                        float *dx, float *partial, float *dwh_dbh, void *stream);
 
 /* Masked MSE: grad = 2 m (pred - target) inv_count (mask per pixel of 2
  * values, may be NULL); per-block loss partials (glide_unet_mse_blocks(n)). */
@@ -89,19 +90,24 @@ int glide_unet_adam(float *w, const float *g, float *m, float *v, long n, float 
                     float eps, int step, float weight_decay, void *stream);
 
 /* Network input of one domain (ny x nx cells, FP64 fields) as BF16 NHWC with
- * 32 channels: H/1000, bed/1000, 100 |grad s|, log(beta) + 2, B/24 - 1, then
- * zeros; s = max(bed, -0.917 H) + H (glide/physics.py:223-229). */
+ * 32 channels: H/1000, tanh((bed + 0.917 H)/200), 100 |grad s|, log(beta) + 2,
+ * B/24 - 1, 100 ds/dx, 100 ds/dy, then zeros; s = max(bed, -0.917 H) + H
+ * (glide/physics.py:223-229). */
 int glide_unet_features(const double *H, const double *bed, const double *beta, const double *B, int ny,
                         int nx, double dx, uint16_t *x, void *stream);
 
-/* Cell-centred (u, v) targets x scale (FP32, 2 per cell) from MAC faces. */
-int glide_unet_targets(const double *u, const double *v, int ny, int nx, double scale, float *tgt,
-                       void *stream);
+/* Training targets, 3 FP32 values per cell: cell-centred u and v times
+ * scale, and the step's thickness change (H - H_prev) times hscale. */
+int glide_unet_targets(const double *u, const double *v, const double *H, const double *H_prev, int ny,
+                       int nx, double scale, double hscale, float *tgt, void *stream);
 
-/* MAC faces from cell-centred predictions / scale: interior faces average
- * their two cells, boundary faces copy their one cell. */
-int glide_unet_to_faces(const float *pred, int ny, int nx, double scale, double *u, double *v,
-                        void *stream);
+/* Solver state from a prediction (3 per cell): MAC faces from the
+ * cell-centred velocities / scale (interior faces average their two cells,
+ * boundary faces copy their one cell) and H = max(H_prev + dH / hscale,
+ * gamma). */
+int glide_unet_apply_prediction(const float *pred, int ny, int nx, double scale, double hscale, double *u,
+                                double *v, double *H, const double *H_prev, const double *gamma,
+                                void *stream);
 
 #ifdef __cplusplus
 }

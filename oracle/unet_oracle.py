@@ -10,7 +10,7 @@ reference file to follow; the layer algebra is the textbook one:
   conv3x3  y[p][co] = act(b[co] + sum_{t,ci} W[t][co][ci] x[p+off_t][ci])
   dgrad    dx[q][ci] = sum_{t,co} dz[q-off_t][co] W[t][co][ci]
   wgrad    dW[t][co][ci] = sum_p dz[p][co] x[p+off_t][ci]
-  pool2    2x2 mean;  up2  nearest x2;  head  1x1 linear (FP32)
+  pool2    2x2 mean;  up2  nearest x2;  head  1x1 linear, 3 outputs (FP32)
 
 With ``rounding=False`` every rounding point is the identity, which the
 finite-difference tests use to pin the backward formulas.  Pins: tests/
@@ -125,6 +125,7 @@ POOLS = {"p1": "e1", "p2": "e2", "p3": "e3"}      # pooled tensor <- source
 UPS = {"u3": "b", "u2": "d3", "u1": "d2"}         # upsampled tensor <- source
 IN_CH = 32
 HEAD_IN = "d1"
+N_OUT = 3          # u, v, H - H_prev
 
 
 def channels():
@@ -158,8 +159,8 @@ def init_params(seed: int = 0) -> dict:
     for name, (co, ci) in layer_shapes().items():
         p[name + ".W"] = (rng.standard_normal((9, co, ci)) * np.sqrt(2.0 / (9 * ci))).astype(np.float32)
         p[name + ".b"] = np.zeros(co, dtype=np.float32)
-    p["head.W"] = (rng.standard_normal((2, 32)) * np.sqrt(1.0 / 32)).astype(np.float32)
-    p["head.b"] = np.zeros(2, dtype=np.float32)
+    p["head.W"] = (rng.standard_normal((N_OUT, 32)) * np.sqrt(1.0 / 32)).astype(np.float32)
+    p["head.b"] = np.zeros(N_OUT, dtype=np.float32)
     return p
 
 
@@ -200,7 +201,7 @@ def backward(params: dict, acts: dict, dpred: np.ndarray, rounding: bool = True)
     grads = {}
     d1 = acts[HEAD_IN]
     grads["head.W"] = np.einsum("nhwo,nhwc->oc", dpred, d1.astype(np.float64))
-    grads["head.b"] = dpred.reshape(-1, 2).sum(axis=0)
+    grads["head.b"] = dpred.reshape(-1, dpred.shape[-1]).sum(axis=0)
     g[HEAD_IN] = dpred @ params["head.W"].astype(np.float64)
     shapes = layer_shapes()
     ch = channels()

@@ -152,7 +152,7 @@ def test_network_loss_and_gradients_match_oracle():
     params = net.state_dict()
     N, H, W = 2, 16, 24
     x0 = bf16_rand(rng, (N, H, W, WS.IN_CH))
-    tgt = rng.standard_normal((N, H, W, 2)).astype(np.float32)
+    tgt = rng.standard_normal((N, H, W, WS.N_OUT)).astype(np.float32)
     xt, tt = bits_t(x0), f32_t(tgt)
     _KEEP.extend([xt, tt])
     pred = net.forward(xt, N, H, W, train=True)
@@ -162,7 +162,7 @@ def test_network_loss_and_gradients_match_oracle():
     rp, acts = U.forward(params, x0)
     rl, dpred = U.mse(rp, tgt)
     rg = U.backward(params, acts, dpred)
-    gp = pred.cpu().numpy().reshape(N, H, W, 2)
+    gp = pred.cpu().numpy().reshape(N, H, W, WS.N_OUT)
     assert np.linalg.norm(gp - rp) <= 1e-2 * np.linalg.norm(rp)
     assert abs(loss - rl) <= 1e-2 * rl
     worst = {}
@@ -184,7 +184,7 @@ def test_training_reduces_loss():
         x[n, :, :, 0] = a * yy + b * xx
         x[n, :, :, 1] = np.sin(3 * xx + n)
     x = U.bf16(x)
-    tgt = np.stack([x[..., 0] * 2 + x[..., 1], x[..., 0] - x[..., 1] ** 2], -1).astype(np.float32)
+    tgt = np.stack([x[..., 0] * 2 + x[..., 1], x[..., 0] - x[..., 1] ** 2, x[..., 1] * 0.5], -1).astype(np.float32)
     net = WS.UNet(DEV, seed=1)
     xt, tt = bits_t(x), f32_t(tgt)
     losses = []

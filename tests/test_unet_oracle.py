@@ -79,7 +79,7 @@ def test_network_gradients_match_finite_differences():
         if k.endswith(".b"):
             params[k] = rng.standard_normal(params[k].shape) * 0.1
     x0 = rng.standard_normal((1, 8, 8, U.IN_CH))
-    tgt = rng.standard_normal((1, 8, 8, 2))
+    tgt = rng.standard_normal((1, 8, 8, U.N_OUT))
 
     def loss(p):
         pred, _ = U.forward(p, x0, rounding=False)
