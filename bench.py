@@ -1,13 +1,15 @@
 """Benchmark of the FP64 SSA / FASCD forward step on one MI355X.
 
 Headline (the "value"): ms per forward step to 1e-8 relative residual at
-512 x 512, dx = 1500 m, 5 levels, dt = 10 a, on seeded ice sheets
+512 x 512, dx = 1500 m, 5 levels, dt = 1 a, on seeded ice sheets
 (glide_amd/synthetic.py ice_sheet, test seeds 20000...), starting from
 u = v = 0 with H = H_prev, iterating FASCD V-cycles until the reference's
 residual norm (glide/physics.py:176-203) is <= 1e-8 x the cold start's
 initial norm.  Plain FASCD and the learned warm start (the trained network
 of glide_amd/data/warmstart_unet.npz; inference inside the timed region)
 are timed on the same seeds; value = the learned time (plain if no weights).
+(dt = 1 a: with dt = 10 a plain FASCD stalls above 1e-8 on most 512^2 sheets
+of this family, so a time to 1e-8 does not exist there.)
 One "step" = one seed's forward step; inputs are resident in HBM.
 
 Also reported on the same JSON line:
@@ -203,12 +205,13 @@ def ensemble_bench(dt=10.0, n_vcycles=3):
 
 WEIGHTS = os.path.join(ROOT, "glide_amd", "data", "warmstart_unet.npz")
 TOL = 1e-8
+DT = 1.0          # a: at 10 a plain FASCD stalls above 1e-8 on most 512^2 sheets of this family
 N512 = 512
 DX512 = 1500.0
 LEVELS512 = 5
 
 
-def to_tolerance_bench(steps: int, warmup: int, dt=10.0):
+def to_tolerance_bench(steps: int, warmup: int, dt=DT):
     """One forward step to ||r|| <= 1e-8 ||r0_cold|| on seeded 512^2 ice
     sheets (test seeds 20000...), plain FASCD from u = v = 0 against the
     learned warm start (network inference included in the timed region)."""
@@ -248,7 +251,7 @@ def to_tolerance_bench(steps: int, warmup: int, dt=10.0):
     return rows, last, net
 
 
-def learned_batched_bench(net, n_members=16, dt=10.0):
+def learned_batched_bench(net, n_members=16, dt=DT):
     """16 seeded 512^2 sheets as one ensemble: plain batched FASCD vs batched
     network inference + batched FASCD, both to 1e-8 of each member's cold
     initial residual."""
@@ -321,7 +324,7 @@ def reference_schedule_bench(dt=10.0, steps=3):
             "ms_per_step": round(ms, 3), "rel_residual_after_step": [float(f"{x:.3e}") for x in ph.last_history]}
 
 
-def cpu_baseline_tol(seed: int, vcycles: int, threads: int, dt=10.0):
+def cpu_baseline_tol(seed: int, vcycles: int, threads: int, dt=DT):
     """The CPU oracle (OpenMP) on the headline workload: one V-cycle of the
     512^2 sheet from the cold start timed, times the plain V-cycle count."""
     from glide_amd import synthetic as S
@@ -374,7 +377,7 @@ def main():
         "value": round(value, 3), "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded ice sheets, glide_amd/synthetic.py ice_sheet, test seeds 20000+)",
-        "config": {"workload": f"ice_sheet_512: {N512}x{N512}, dx={DX512:.0f} m, {LEVELS512} levels, dt=10 a, one "
+        "config": {"workload": f"ice_sheet_512: {N512}x{N512}, dx={DX512:.0f} m, {LEVELS512} levels, dt={DT:g} a, one "
                                f"forward step from u = v = 0 to ||r|| <= 1e-8 ||r0_cold||"
                                + (" with the learned warm start" if learned else " (no trained weights: plain)"),
                    "global_batch": 1, "parallelism": "single GPU"},

@@ -3,8 +3,8 @@
     python -m glide_amd.train_warmstart --n-train 2000 --out glide_amd/data/warmstart_unet.npz
 
 1. Data: ``n_train`` seeded ice sheets (glide_amd.synthetic.ice_sheet, seeds
-   10000...), ``size`` x ``size`` cells, dx alternating 1000 / 1500 m, each
-   solved by batched FASCD (Ensemble, ``batch`` members per plan) from a cold
+   10000...), ``size`` x ``size`` cells, dx alternating 1000 / 1500 m, one
+   forward step of ``dt`` (1 a) each, solved by batched FASCD (Ensemble, ``batch`` members per plan) from a cold
    start (u = v = 0) to ``tol`` relative residual; the network input is the
    member's cell fields (glide_unet_features) and the target its converged,
    cell-centred (u, v) and thickness change H - H_prev; solves that do not reach ``tol`` in ``max_vcycles``
@@ -47,7 +47,7 @@ def member(seed: int, size: int, dx: float | None = None, n_levels: int | None =
     return ph
 
 
-def generate(n: int, size: int, seed0: int = TRAIN_SEED0, dt: float = 10.0, tol: float = 1e-8, batch: int = 64,
+def generate(n: int, size: int, seed0: int = TRAIN_SEED0, dt: float = 1.0, tol: float = 1e-8, batch: int = 64,
              max_vcycles: int = 80, device=None, log=print, drop_unconverged: bool = True):
     """Solve n seeded sheets; returns (X int16 [m, size*size*32], Y fp32 [m, size*size*3], stats),
     keeping only the m solves that reached ``tol`` unless ``drop_unconverged`` is False."""
@@ -126,7 +126,7 @@ def evaluate_loss(net: UNet, X, Y, size, batch):
     return tot / max(cnt, 1)
 
 
-def evaluate(net: UNet, seeds, size: int, dx: float | None = None, n_levels: int | None = None, dt: float = 10.0,
+def evaluate(net: UNet, seeds, size: int, dx: float | None = None, n_levels: int | None = None, dt: float = 1.0,
              tol: float = 1e-8, max_vcycles: int = 200, device=None) -> list[dict]:
     """Cold vs warm-started forward step on each seed: V-cycles and wall time
     to ||r|| <= tol * r0_cold, and the prediction's error vs the cold solve."""
@@ -191,8 +191,9 @@ def main():
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--dt", type=float, default=1.0, help="time step of the solved forward steps (a)")
     ap.add_argument("--n-test", type=int, default=4)
-    ap.add_argument("--test-size", type=int, nargs="*", default=[256, 1024])
+    ap.add_argument("--test-size", type=int, nargs="*", default=[256, 512, 1024])
     ap.add_argument("--out", default="warmstart_unet.npz")
     ap.add_argument("--report", default=None)
     args = ap.parse_args()
@@ -201,23 +202,23 @@ def main():
         print(msg, flush=True)
 
     log(f"== generating {args.n_train} solves at {args.size}^2")
-    X, Y, gen = generate(args.n_train, args.size, tol=args.tol, log=log)
+    X, Y, gen = generate(args.n_train, args.size, tol=args.tol, dt=args.dt, log=log)
     log(f"   {gen}")
     net = UNet(seed=0)
     log(f"== training ({args.epochs} epochs, batch {args.batch})")
     tr = train(net, X, Y, args.size, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
     del X, Y
     torch.cuda.empty_cache()
-    net.save(args.out, vel_scale=VEL_SCALE, dh_scale=DH_SCALE, train_size=args.size, n_train=args.n_train, epochs=args.epochs,
+    net.save(args.out, vel_scale=VEL_SCALE, dh_scale=DH_SCALE, dt=args.dt, train_size=args.size, n_train=args.n_train, epochs=args.epochs,
              final_val_loss=tr["history"][-1]["val_loss"])
     report = {"generate": gen, "train": tr}
     for size in args.test_size:
-        levels = 6 if size == 1024 else None
-        dxs = [1000.0] if size == 1024 else [None]
+        levels = 6 if size == 1024 else (5 if size == 512 else None)
+        dxs = [1000.0] if size == 1024 else ([1500.0] if size == 512 else [None])
         for dx in dxs:
             log(f"== evaluating at {size}^2")
             rows = evaluate(net, range(TEST_SEED0, TEST_SEED0 + args.n_test), size, dx=dx, n_levels=levels,
-                            tol=args.tol)
+                            tol=args.tol, dt=args.dt)
             for r in rows:
                 log(f"   {r}")
             report[f"eval_{size}"] = {"rows": rows, "summary": summarize(rows)}

@@ -41,7 +41,7 @@ UPS = {"u3": "b", "u2": "d3", "u1": "d2"}
 IN_CH = 32
 HEAD_IN = "d1"
 VEL_SCALE = 0.01          # network units: m/a x 0.01
-DH_SCALE = 0.01           # network units of the step's thickness change: m x 0.01
+DH_SCALE = 0.05           # network units of the step's thickness change: m x 0.05
 N_OUT = 3                 # outputs per cell: u, v, H - H_prev
 DOWNSAMPLE = 8            # domain sides must be multiples of this
 
