@@ -36,7 +36,6 @@ using namespace glide;
 
 namespace {
 
-constexpr int NXCD = 8;           // XCDs per MI355X
 constexpr int BTX = 64;           // tile width: one wave64 per tile row
 constexpr int PTY = 4;            // tile height of the element-wise passes
 constexpr int NSHAPE = 5;         // tile heights TY = 1, 2, 4, 8, 16
@@ -82,21 +81,6 @@ __device__ __forceinline__ int find_entry(const int *__restrict__ pre, int n, in
     }
     return lo;
 }
-
-// Persistent tile loop: workgroup b (on XCD b % 8) walks its XCD's contiguous
-// eighth of the tiles.  The host launches at least 8 workgroups.
-template <class Body>
-__device__ __forceinline__ void tile_loop(int total, const Body &body)
-{
-    const int G = gridDim.x, b = blockIdx.x;
-    const int x = b % NXCD, lane = b / NXCD;
-    const int nwg = (G - x + NXCD - 1) / NXCD;
-    const int span = (total + NXCD - 1) / NXCD;
-    const int lo = x * span, hi = min(total, lo + span);
-    for (int t = lo + lane; t < hi; t += nwg) body(t);
-}
-
-__host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ---------------------------------------------------------------------------
 // batched kernels

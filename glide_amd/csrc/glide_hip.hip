@@ -54,6 +54,33 @@ int check_geom(const GlideGeom *g)
 
 inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
+// Workgroups per CU of the persistent single-level launches (Vanka sweep,
+// residual); set by glide_set_persistence.  The Vanka sweep's 64 gives one
+// tile per workgroup at every level size used here (measured fastest: the
+// hardware then balances clustered local-smoother work); the residual's 4
+// loops each workgroup over several tiles.
+int g_wg_per_cu[2] = {64, 4};
+
+int n_cu()
+{
+    static int cu[16] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return 256;
+    if (!cu[dev]) {
+        hipDeviceProp_t prop;
+        cu[dev] = hipGetDeviceProperties(&prop, dev) == hipSuccess ? prop.multiProcessorCount : 256;
+    }
+    return cu[dev];
+}
+// min(tiles, CUs x per_cu) workgroups, at least 8 and a multiple of 8 (tile_loop)
+dim3 persistent_grid(long tiles, int per_cu)
+{
+    long g = tiles < (long)n_cu() * per_cu ? tiles : (long)n_cu() * per_cu;
+    g = g < NXCD ? NXCD : g;
+    return dim3((unsigned)((g + NXCD - 1) / NXCD * NXCD));
+}
+dim3 vanka_grid(int ny, int nx) { return persistent_grid((long)cdiv(nx, TX) * cdiv(ny, TY), g_wg_per_cu[0]); }
+dim3 residual_grid(int ny, int nx) { return persistent_grid((long)cdiv(nx + 1, TX) * cdiv(ny + 1, TY), g_wg_per_cu[1]); }
 dim3 point_grid(int ny, int nx) { return dim3((nx + 1 + TX - 1) / TX, (ny + 1 + TY - 1) / TY); }
 dim3 cell_grid(int ny, int nx) { return dim3((nx + TX - 1) / TX, (ny + TY - 1) / TY); }
 inline unsigned blocks1d(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
@@ -73,7 +100,11 @@ __global__ __launch_bounds__(NT) void k_residual(GlideGeom g, Params p, double *
                                                  const double *__restrict__ mask, const double *__restrict__ gamma)
 {
     __shared__ ResidLDS<TX, TY> S;
-    residual_tile_lds<TX, TY>(g, p, r, U, f, sub, bed, B, beta, mask, gamma, blockIdx.x, blockIdx.y, S);
+    const int tx = cdiv(g.nx + 1, TX), total = tx * cdiv(g.ny + 1, TY);
+    tile_loop(total, [&](int t) {
+        __syncthreads();   // the previous tile's readers of S are done
+        residual_tile_lds<TX, TY>(g, p, r, U, f, sub, bed, B, beta, mask, gamma, t % tx, t / tx, S);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -89,7 +120,11 @@ __global__ __launch_bounds__(NT) void k_vanka(GlideGeom g, Params p, const doubl
                                               int nbx, int n_newton, double *__restrict__ cd)
 {
     __shared__ double eta_s[TY + 2][TX + 2];
-    vanka_tile<TX, TY, LOCAL>(g, p, U, f, bed, B, beta, gamma, blk, nbx, n_newton, cd, blockIdx.x, blockIdx.y, eta_s);
+    const int tx = cdiv(g.nx, TX), total = tx * cdiv(g.ny, TY);
+    tile_loop(total, [&](int t) {
+        __syncthreads();   // the previous tile's readers of eta_s are done
+        vanka_tile<TX, TY, LOCAL>(g, p, U, f, bed, B, beta, gamma, blk, nbx, n_newton, cd, t % tx, t / tx, eta_s);
+    });
 }
 
 
@@ -783,7 +818,7 @@ int launch_residual(const GlideGeom &g, const Params &p, double *r, const double
                     const double *bed, const double *B, const double *beta, const double *mask, const double *gamma,
                     hipStream_t st)
 {
-    hipLaunchKernelGGL(k_residual, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, r, U, f, sub, bed, B, beta,
+    hipLaunchKernelGGL(k_residual, residual_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, r, U, f, sub, bed, B, beta,
                        mask, gamma);
     return check_launch("residual");
 }
@@ -798,12 +833,12 @@ int launch_sweeps(const GlideGeom &g, const Params &p, double *U, const double *
     const int nbx = nbx_of(g.nx);
     for (int s = 0; s < n_sweeps; s++) {
         if (blk) {
-            hipLaunchKernelGGL(k_vanka<true>, cell_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B, beta,
+            hipLaunchKernelGGL(k_vanka<true>, vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B, beta,
                                gamma, blk, nbx, n_newton, scratch);
             hipLaunchKernelGGL(k_vanka_apply<true>, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g.ny, g.nx, U,
                                scratch, blk, nbx);
         } else {
-            hipLaunchKernelGGL(k_vanka<false>, cell_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B, beta,
+            hipLaunchKernelGGL(k_vanka<false>, vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B, beta,
                                gamma, blk, nbx, n_newton, scratch);
             hipLaunchKernelGGL(k_vanka_apply<false>, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g.ny, g.nx, U,
                                scratch, blk, nbx);
@@ -991,6 +1026,14 @@ int adjoint_vc(const GlideLevel *L, int n, int k, const GlideSchedule &sc, hipSt
 extern "C" {
 
 const char *glide_last_error(void) { return g_last_error.c_str(); }
+int glide_set_persistence(int kernel, int wg_per_cu)
+{
+    if (kernel < 0 || kernel > 1) return fail(GLIDE_E_ARG, "kernel must be 0 (Vanka sweep) or 1 (residual)");
+    if (wg_per_cu < 1 || wg_per_cu > 64) return fail(GLIDE_E_ARG, "workgroups per CU must be in [1, 64]");
+    g_wg_per_cu[kernel] = wg_per_cu;
+    return GLIDE_OK;
+}
+
 const char *glide_version(void) { return "glide_amd 0.1 (gfx950, fp64)"; }
 
 int glide_residual(const GlideGeom *g, const GlideParams *p, double *r, const double *U, const double *f,

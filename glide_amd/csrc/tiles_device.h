@@ -16,11 +16,45 @@ namespace glide {
 int set_error(int code, const char *msg);
 
 constexpr int RB = 14;  // reference launch-block interior (glide/grid.py:184-186)
+constexpr int NXCD = 8; // XCDs per MI355X
+
+__host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// Persistent tile loop.  With at least as many workgroups as tiles each
+// workgroup runs tile blockIdx.x (the hardware dispatcher balances them).
+// Otherwise workgroups loop over tiles: workgroups are dispatched round-robin
+// over the XCDs (workgroup b runs on XCD b % 8), the tiles are cut into chunks
+// of up to TCH consecutive tiles (neighbours sharing halo rows; smaller when
+// there are few tiles, so every XCD gets at least two chunks) dealt
+// round-robin to the XCDs, and each XCD's workgroups walk its chunks' tiles.
+// Halos then meet in one L2 while work clustered in space (the local
+// smoother's active blocks) still spreads over all XCDs.  The launch must
+// have at least 8 workgroups.
+constexpr int TCH = 8;
 
 __host__ __device__ inline size_t n_u(int ny, int nx) { return (size_t)ny * (nx + 1); }
 __host__ __device__ inline size_t n_v(int ny, int nx) { return (size_t)(ny + 1) * nx; }
 __host__ __device__ inline size_t n_c(int ny, int nx) { return (size_t)ny * nx; }
 __host__ __device__ inline size_t n_t(int ny, int nx) { return n_u(ny, nx) + n_v(ny, nx) + n_c(ny, nx); }
+
+template <class Body>
+__device__ __forceinline__ void tile_loop(int total, const Body &body)
+{
+    const int G = gridDim.x, b = blockIdx.x;
+    if (G >= total) {
+        if (b < total) body(b);
+        return;
+    }
+    const int tch = max(1, min(TCH, total / (2 * NXCD)));
+    const int x = b % NXCD, lane = b / NXCD;
+    const int nwg = (G - x + NXCD - 1) / NXCD;
+    const int nchunks = (total + tch - 1) / tch;
+    const int my_chunks = nchunks > x ? (nchunks - x + NXCD - 1) / NXCD : 0;
+    for (int k = lane; k < my_chunks * tch; k += nwg) {
+        const int t = (x + NXCD * (k / tch)) * tch + k % tch;
+        if (t < total) body(t);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // viscosity tile: cells [i0-1, i0+TY] x [j0-1, j0+TX] into LDS (0 outside)

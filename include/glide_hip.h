@@ -250,6 +250,13 @@ int glide_batch_set_tiles(GlideBatch *batch, int kernel, int phase, int ty, int 
 int glide_batch_time_kernel(GlideBatch *batch, int kernel, int phase, int ty, int wg_per_cu,
                             int reps, double *ms, void *stream);
 
+/* Workgroups per CU of the persistent single-level launches: kernel 0 =
+ * the Vanka sweep (default 64, i.e. one tile per workgroup), 1 = the
+ * residual (default 4).  Each launch has
+ * min(tiles, CUs x wg_per_cu) workgroups looping over its tiles; results do
+ * not depend on the setting.  Process-wide; no reference counterpart. */
+int glide_set_persistence(int kernel, int wg_per_cu);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *glide_last_error(void);
 
