@@ -47,9 +47,21 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-member batched ensemble")
+    ap.add_argument("--reference-only", action="store_true",
+                    help="only the cap_512 3-V-cycle steps (for profiling the reference schedule)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the kernel microbenchmarks (for profiling them alone)")
     return ap.parse_args()
+
+
+# HBM bytes per launch at 512^2, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (one run each) and calibrated in the same program on kernels of
+# known traffic: a 256 MiB torch.sum reads 131,134 KiB of FETCH_SIZE (bytes =
+# 2048 x FETCH_SIZE) and a 256 MiB fill writes 262,144 KiB of WRITE_SIZE
+# (bytes = 1024 x WRITE_SIZE).  Vanka sweep + apply pass: FETCH 13,166 + 8,713.5,
+# WRITE 10,240 + 6,248; residual: FETCH 16,071.6, WRITE 6,175.
+TRAFFIC_512 = {"vanka": (2 * (13166.0 + 8713.5) + 10240.0 + 6248.0) * 1024,
+               "residual": (2 * 16071.6 + 6175.0) * 1024}
 
 
 def kernel_rooflines(ph, reps=20):
@@ -102,12 +114,14 @@ def kernel_rooflines(ph, reps=20):
         "roofline": {"kernel": "k_vanka<false> + k_vanka_apply (finest full sweep, 512^2)", "bound": "mfma",
                      "note": "FP64 VALU arithmetic (no MFMA); peak is MI355X FP64 78.6 TFLOP/s",
                      "achieved": round(vanka_tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(vanka_tf / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "frac": round(vanka_tf / PEAK_FP64_TFLOPS, 4),
+                     "traffic": TRAFFIC_512["vanka"] if (g.ny, g.nx) == (512, 512) else None,
                      "ms_per_launch": round(out["vanka"], 4),
                      "work": f"{nc} cells x (eta + 10 Newton x {FLOPS_NEWTON_ITER} flop)"},
         "stencil_roofline": {"kernel": "k_residual (512^2)", "bound": "hbm", "achieved": round(res_gbs, 1),
                              "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(res_gbs / PEAK_HBM_GBS, 4),
-                             "traffic": None, "ms_per_launch": round(out["residual"], 4),
+                             "traffic": TRAFFIC_512["residual"] if (g.ny, g.nx) == (512, 512) else None,
+                             "ms_per_launch": round(out["residual"], 4),
                              "work": f"{res_bytes} algorithmic bytes"},
     }
 
@@ -167,6 +181,12 @@ def ensemble_bench(dt=10.0, n_vcycles=3):
         ph.forward(dt, n_vcycles=n_vcycles)
         torch.cuda.synchronize()
         solo_ms.append((time.perf_counter() - t0) * 1e3)
+    # the batched Vanka sweep with every member enabled: phase 0 holds the
+    # 6-level members' finest levels (16 x 512^2 + 16 x 768^2)
+    vty = int(rec["choice"]["vanka"]["0"])
+    v_ms = ens.time_kernel(BK_VANKA, 0, vty, 0, reps=5)
+    v_cells = sum(d["ny"] * d["nx"] for d in members if d["n_levels"] == ens.n_phases)
+    v_tf = sweep_flops(v_cells) / (v_ms * 1e-3) / 1e12
     # ensemble-wide residual on every finest level (one launch)
     fin = ens.n_phases
     ty = int(rec["choice"]["residual"][str(fin)])
@@ -192,6 +212,11 @@ def ensemble_bench(dt=10.0, n_vcycles=3):
         "throughput_member_steps_per_s": round(len(members) / (batched_ms * 1e-3), 2),
         "throughput_cell_vcycles_per_s": float(f"{cells * n_vcycles / (batched_ms * 1e-3):.4e}"),
         "autotune_s": round(tune_s, 1), "tile_heights": rec["choice"],
+        "vanka_roofline": {"kernel": "kb_vanka (phase 0: 16 x 512^2 + 16 x 768^2 finest levels, one launch)",
+                           "bound": "mfma", "note": "FP64 VALU arithmetic (no MFMA)", "achieved": round(v_tf, 3),
+                           "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(v_tf / PEAK_FP64_TFLOPS, 4),
+                           "ms_per_launch": round(v_ms, 4), "work": f"{v_cells} cells x (eta + 10 Newton x "
+                                                                    f"{FLOPS_NEWTON_ITER} flop)", "tile_height": vty},
         "stencil_roofline": {"kernel": "kb_residual (every finest level, one launch)", "bound": "hbm",
                              "achieved": round(res_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": round(res_gbs / PEAK_HBM_GBS, 4), "ms_per_launch": round(res_ms, 4),
@@ -354,6 +379,9 @@ def main():
     import numpy as np
     import torch
 
+    if args.reference_only:
+        print(json.dumps(reference_schedule_bench()))
+        return
     if args.kernels_only:
         from glide_amd import synthetic as S
         from glide_amd.physics import IcePhysics
@@ -363,12 +391,14 @@ def main():
         ph.set_geometry(w["bed"], w["H"])
         ph.set_parameters(B=w["B"], beta=w["beta"], smb=w["smb"])
         ph.forward(w["dt"], n_vcycles=1)
-        # a plain copy of known size: the calibration of the HBM byte counters
+        # kernels of known traffic, the calibration of the HBM byte counters:
+        # a fill writes 256 MiB, a full sum reads 256 MiB, a copy does both
         a = torch.ones(1 << 25, dtype=torch.float64, device="cuda")
+        s = a.sum()
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({**kernel_rooflines(ph), "calibration_copy_bytes_each_way": a.numel() * 8}))
+        print(json.dumps({**kernel_rooflines(ph), "calibration_bytes": a.numel() * 8, "sum": float(s)}))
         return
     rows, ph, net = to_tolerance_bench(args.steps, args.warmup)
     plain_ms = float(np.mean([r["plain_ms"] for r in rows]))
