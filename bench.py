@@ -154,7 +154,7 @@ def ensemble_bench(dt=10.0, n_vcycles=3):
 
     from glide_amd import synthetic as S
     from glide_amd.autotune import tune
-    from glide_amd.ensemble import BK_RESIDUAL, Ensemble
+    from glide_amd.ensemble import BK_RESIDUAL, BK_VANKA, Ensemble
     from glide_amd.physics import IcePhysics
 
     members = S.ensemble_members()
