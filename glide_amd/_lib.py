@@ -115,6 +115,7 @@ SIGNATURES = {
     "glide_unet_apply_prediction": (c_int, [c_void_p, c_int, c_int, c_double, c_double, DP, DP, DP, DP, DP,
                                             c_void_p]),
     "glide_set_persistence": (c_int, [c_int, c_int]),
+    "glide_set_row_split": (c_int, [ctypes.c_long]),
     "glide_last_error": (c_char_p, []),
     "glide_version": (c_char_p, []),
 }

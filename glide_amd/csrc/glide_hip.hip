@@ -60,6 +60,8 @@ inline hipStream_t S(void *s) { return (hipStream_t)s; }
 // hardware then balances clustered local-smoother work); the residual's 4
 // loops each workgroup over several tiles.
 int g_wg_per_cu[2] = {64, 4};
+// Levels with at most this many cells use the row-split Vanka sweep.
+long g_row_split_max_cells = 128 * 128;
 
 int n_cu()
 {
@@ -127,6 +129,107 @@ __global__ __launch_bounds__(NT) void k_vanka(GlideGeom g, Params p, const doubl
     });
 }
 
+
+// ---------------------------------------------------------------------------
+// Row-split Vanka sweep for small levels, where a sweep takes as long as one
+// wave's Newton chain.  A workgroup holds 4 waves x 64 cells of one grid row,
+// one wave per SIMD; wave w assembles rows {0}, {1}, {2}, {3, 4} of each
+// cell's 5x5 block (vanka_assemble with a row mask: each entry receives the
+// operations it receives in the full assembly), the rows meet in LDS (double
+// buffered, one barrier per Newton iteration) and every wave solves the
+// block and updates its copy of the unknowns.  Same bits as k_vanka.
+// ---------------------------------------------------------------------------
+template <int ROWS>
+__device__ __forceinline__ void rows_to_lds(const CellNbhd &N, const Params &p, int i, int j, int ny, int nx,
+                                            double dx_inv, double dt, double u_l, double u_r, double v_t, double v_b,
+                                            double H_c, double th, double (*x)[6][64], int lane)
+{
+    double J[25], r[5];
+    vanka_assemble<0, ROWS>(N, p, i, j, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, false, J, r);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        if (ROWS & (1 << q)) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) x[q][c][lane] = J[5 * q + c];
+            x[q][5][lane] = r[q];
+        }
+    }
+}
+
+template <bool LOCAL>
+__global__ __launch_bounds__(256) void k_vanka_rows(GlideGeom g, Params p, const double *__restrict__ U,
+                                                    const double *__restrict__ f, const double *__restrict__ bed,
+                                                    const double *__restrict__ B, const double *__restrict__ beta,
+                                                    const double *__restrict__ gamma,
+                                                    const unsigned char *__restrict__ blk, int nbx, int n_newton,
+                                                    double *__restrict__ cd)
+{
+    __shared__ double eta_s[3][64 + 2];
+    __shared__ double xch[2][5][6][64];
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv;
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int i = blockIdx.y, j0 = blockIdx.x * 64, j = j0 + lane;
+    bool mine = j < nx;   // i < ny by the grid
+    if (LOCAL) {
+        mine = mine && blk[(i / RB) * nbx + j / RB];
+        if (!__syncthreads_or(mine)) return;   // uniform: the whole tile is idle
+    }
+    for (int k = w * 64 + lane; k < 3 * 66; k += 256) {   // viscosity of cells [i-1, i+1] x [j0-1, j0+64]
+        const int a = k / 66, b = k - a * 66;
+        const int ci = i - 1 + a, cj = j0 - 1 + b;
+        eta_s[a][b] = (ci >= 0 && ci < ny && cj >= 0 && cj < nx) ? eta_cell(ci, cj, u, v, B, p, g.dx, ny, nx) : 0.0;
+    }
+    __syncthreads();
+    // lanes past the last column work on the last cell (every wave reaches
+    // every barrier of the Newton loop) and discard the result
+    const int jj = j < nx ? j : nx - 1;
+    auto E = [&](int ci, int cj) { return eta_s[ci - i + 1][cj - j0 + 1]; };
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+    CellNbhd N = load_nbhd(i, jj, ny, nx, nu, nv, u, v, H, f, bed, beta, E);
+    nbhd_grounding(N, p);
+    const double u_l0 = ld_u(u, i, jj, ny, nx), u_r0 = ld_u(u, i, jj + 1, ny, nx);
+    const double v_t0 = ld_v(v, i, jj, ny, nx), v_b0 = ld_v(v, i + 1, jj, ny, nx);
+    const double H_c0 = ld_c(H, i, jj, ny, nx), th = ld_c(gamma, i, jj, ny, nx);
+    double u_l = u_l0, u_r = u_r0, v_t = v_t0, v_b = v_b0, H_c = H_c0;
+    for (int k = 0; k < n_newton; k++) {
+        double (*x)[6][64] = xch[k & 1];
+        if (w == 0)
+            rows_to_lds<1>(N, p, i, jj, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, x, lane);
+        else if (w == 1)
+            rows_to_lds<2>(N, p, i, jj, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, x, lane);
+        else if (w == 2)
+            rows_to_lds<4>(N, p, i, jj, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, x, lane);
+        else
+            rows_to_lds<24>(N, p, i, jj, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, x, lane);
+        // one barrier per iteration: buffer k & 1 is written again at k + 2,
+        // after every wave has passed barrier k + 1 and so finished reading it
+        __syncthreads();
+        double J[25], r[5], d[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) J[5 * q + c] = x[q][c][lane];
+            r[q] = x[q][5][lane];
+        }
+        lu5_solve(J, r, d);
+        u_l -= 0.25 * d[0];
+        u_r -= 0.25 * d[1];
+        v_t -= 0.25 * d[2];
+        v_b -= 0.25 * d[3];
+        H_c -= 0.25 * d[4];
+        H_c = fmax(H_c, th);
+    }
+    if (w == 0 && mine) {
+        const size_t ci = (size_t)i * nx + j;
+        cd[ci] = 0.5 * (u_l - u_l0);
+        cd[nc + ci] = 0.5 * (u_r - u_r0);
+        cd[2 * nc + ci] = 0.5 * (v_t - v_t0);
+        cd[3 * nc + ci] = 0.5 * (v_b - v_b0);
+        cd[4 * nc + ci] = H_c - H_c0;
+    }
+}
 
 // Local block (J, r) of every cell after k Newton updates (test hook):
 // out[30*ci + 0..24] = J, out[30*ci + 25..29] = r.
@@ -831,7 +934,23 @@ int launch_sweeps(const GlideGeom &g, const Params &p, double *U, const double *
                   double *scratch, hipStream_t st)
 {
     const int nbx = nbx_of(g.nx);
+    const bool rows = (long)g.ny * g.nx <= g_row_split_max_cells;
+    const dim3 rgrid(cdiv(g.nx, 64), g.ny), rblock(64, 4);
     for (int s = 0; s < n_sweeps; s++) {
+        if (rows) {
+            if (blk) {
+                hipLaunchKernelGGL(k_vanka_rows<true>, rgrid, rblock, 0, st, g, p, U, f, bed, B, beta, gamma, blk, nbx,
+                                   n_newton, scratch);
+                hipLaunchKernelGGL(k_vanka_apply<true>, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g.ny, g.nx, U,
+                                   scratch, blk, nbx);
+            } else {
+                hipLaunchKernelGGL(k_vanka_rows<false>, rgrid, rblock, 0, st, g, p, U, f, bed, B, beta, gamma, blk,
+                                   nbx, n_newton, scratch);
+                hipLaunchKernelGGL(k_vanka_apply<false>, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g.ny, g.nx, U,
+                                   scratch, blk, nbx);
+            }
+            continue;
+        }
         if (blk) {
             hipLaunchKernelGGL(k_vanka<true>, vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B, beta,
                                gamma, blk, nbx, n_newton, scratch);
@@ -1026,6 +1145,13 @@ int adjoint_vc(const GlideLevel *L, int n, int k, const GlideSchedule &sc, hipSt
 extern "C" {
 
 const char *glide_last_error(void) { return g_last_error.c_str(); }
+int glide_set_row_split(long max_cells)
+{
+    if (max_cells < 0) return fail(GLIDE_E_ARG, "max_cells must be >= 0");
+    g_row_split_max_cells = max_cells;
+    return GLIDE_OK;
+}
+
 int glide_set_persistence(int kernel, int wg_per_cu)
 {
     if (kernel < 0 || kernel > 1) return fail(GLIDE_E_ARG, "kernel must be 0 (Vanka sweep) or 1 (residual)");

@@ -70,7 +70,9 @@ __host__ __device__ __forceinline__ void nbhd_grounding(CellNbhd &N, const Param
 
 // MODE 0: forward Vanka (residual, active-set test, diagonal shift)
 // MODE 1: adjoint block (no residual, mass row replaced where adj_masked)
-template <int MODE>
+// ROWS: bit k set = assemble row k (J[5k..5k+4], r[k]); each entry receives
+// exactly the operations it receives in the full assembly (ROWS = 31).
+template <int MODE, int ROWS = 31>
 __host__ __device__ __forceinline__ void vanka_assemble(const CellNbhd &N, const Params &p, int i, int j, int ny, int nx,
                                                double dx_inv, double dt, double u_l, double u_r, double v_t,
                                                double v_b, double H_c, double thklim, bool adj_masked,
@@ -82,157 +84,157 @@ __host__ __device__ __forceinline__ void vanka_assemble(const CellNbhd &N, const
     for (int k = 0; k < 5; k++) r[k] = 0.0;
     const Ground g_c = ground(N.bed_c, H_c, p);   // the only sigmoid that depends on the iterate
     if (MODE == 0) {
-        r[0] -= N.f0;
-        r[1] -= N.f1;
-        r[2] -= N.f2;
-        r[3] -= N.f3;
-        r[4] -= N.f4;
+        if (ROWS & 1) r[0] -= N.f0;
+        if (ROWS & 2) r[1] -= N.f1;
+        if (ROWS & 4) r[2] -= N.f2;
+        if (ROWS & 8) r[3] -= N.f3;
+        if (ROWS & 16) r[4] -= N.f4;
     }
     {
-        J[24] = 1.0 / dt;
-        r[4] += H_c / dt;
+        if (ROWS & 16) J[24] = 1.0 / dt;
+        if (ROWS & 16) r[4] += H_c / dt;
         Calv jc = calving_g(H_c, g_c, p);
-        J[24] -= jc.d_H;
-        r[4] -= jc.res;
+        if (ROWS & 16) J[24] -= jc.d_H;
+        if (ROWS & 16) r[4] -= jc.res;
         Flux q = flux_x(u_l, N.H_l, H_c, j, nx);
-        J[20] -= q.d_u * dx_inv;
-        J[24] -= q.d_H_r * dx_inv;
-        r[4] -= q.res * dx_inv;
+        if (ROWS & 16) J[20] -= q.d_u * dx_inv;
+        if (ROWS & 16) J[24] -= q.d_H_r * dx_inv;
+        if (ROWS & 16) r[4] -= q.res * dx_inv;
         q = flux_x(u_r, H_c, N.H_r, j + 1, nx);
-        J[21] += q.d_u * dx_inv;
-        J[24] += q.d_H_l * dx_inv;
-        r[4] += q.res * dx_inv;
+        if (ROWS & 16) J[21] += q.d_u * dx_inv;
+        if (ROWS & 16) J[24] += q.d_H_l * dx_inv;
+        if (ROWS & 16) r[4] += q.res * dx_inv;
         q = flux_y(v_t, N.H_t, H_c, i, ny);
-        J[22] += q.d_u * dx_inv;
-        J[24] += q.d_H_r * dx_inv;
-        r[4] += q.res * dx_inv;
+        if (ROWS & 16) J[22] += q.d_u * dx_inv;
+        if (ROWS & 16) J[24] += q.d_H_r * dx_inv;
+        if (ROWS & 16) r[4] += q.res * dx_inv;
         q = flux_y(v_b, H_c, N.H_b, i + 1, ny);
-        J[23] -= q.d_u * dx_inv;
-        J[24] -= q.d_H_l * dx_inv;
-        r[4] -= q.res * dx_inv;
-        bool constrained = (MODE == 0) ? ((H_c - dt * r[4]) <= thklim) : adj_masked;
-        if (constrained) {
+        if (ROWS & 16) J[23] -= q.d_u * dx_inv;
+        if (ROWS & 16) J[24] -= q.d_H_l * dx_inv;
+        if (ROWS & 16) r[4] -= q.res * dx_inv;
+        bool constrained = (ROWS & 16) && ((MODE == 0) ? ((H_c - dt * r[4]) <= thklim) : adj_masked);
+        if ((ROWS & 16) && constrained) {
 #pragma unroll
             for (int k = 0; k < 5; k++) J[20 + k] = 0.0;
-            J[24] = 1.0;
-            if (MODE == 0) r[4] = H_c - thklim;
+            if (ROWS & 16) J[24] = 1.0;
+            if (MODE == 0) if (ROWS & 16) r[4] = H_c - thklim;
         }
     }
     {
         double eH = H_c * N.e_c;
         SigN s = sigma_xx(u_l, u_r, v_t, v_b, eH, dx_inv, j, nx);
-        r[0] += s.res * dx_inv;
-        J[0] += s.d_u_l * dx_inv; J[1] += s.d_u_r * dx_inv; J[2] += s.d_v_t * dx_inv;
-        J[3] += s.d_v_b * dx_inv; J[4] += s.d_eta_H * N.e_c * dx_inv;
-        r[1] -= s.res * dx_inv;
-        J[5] -= s.d_u_l * dx_inv; J[6] -= s.d_u_r * dx_inv; J[7] -= s.d_v_t * dx_inv;
-        J[8] -= s.d_v_b * dx_inv; J[9] -= s.d_eta_H * N.e_c * dx_inv;
+        if (ROWS & 1) r[0] += s.res * dx_inv;
+        if (ROWS & 1) J[0] += s.d_u_l * dx_inv; if (ROWS & 1) J[1] += s.d_u_r * dx_inv; if (ROWS & 1) J[2] += s.d_v_t * dx_inv;
+        if (ROWS & 1) J[3] += s.d_v_b * dx_inv; if (ROWS & 1) J[4] += s.d_eta_H * N.e_c * dx_inv;
+        if (ROWS & 2) r[1] -= s.res * dx_inv;
+        if (ROWS & 2) J[5] -= s.d_u_l * dx_inv; if (ROWS & 2) J[6] -= s.d_u_r * dx_inv; if (ROWS & 2) J[7] -= s.d_v_t * dx_inv;
+        if (ROWS & 2) J[8] -= s.d_v_b * dx_inv; if (ROWS & 2) J[9] -= s.d_eta_H * N.e_c * dx_inv;
         SigN t = sigma_yy(u_l, u_r, v_t, v_b, eH, dx_inv, i, ny);
-        r[2] -= t.res * dx_inv;
-        J[10] -= t.d_u_l * dx_inv; J[11] -= t.d_u_r * dx_inv; J[12] -= t.d_v_t * dx_inv;
-        J[13] -= t.d_v_b * dx_inv; J[14] -= t.d_eta_H * N.e_c * dx_inv;
-        r[3] += t.res * dx_inv;
-        J[15] += t.d_u_l * dx_inv; J[16] += t.d_u_r * dx_inv; J[17] += t.d_v_t * dx_inv;
-        J[18] += t.d_v_b * dx_inv; J[19] += t.d_eta_H * N.e_c * dx_inv;
+        if (ROWS & 4) r[2] -= t.res * dx_inv;
+        if (ROWS & 4) J[10] -= t.d_u_l * dx_inv; if (ROWS & 4) J[11] -= t.d_u_r * dx_inv; if (ROWS & 4) J[12] -= t.d_v_t * dx_inv;
+        if (ROWS & 4) J[13] -= t.d_v_b * dx_inv; if (ROWS & 4) J[14] -= t.d_eta_H * N.e_c * dx_inv;
+        if (ROWS & 8) r[3] += t.res * dx_inv;
+        if (ROWS & 8) J[15] += t.d_u_l * dx_inv; if (ROWS & 8) J[16] += t.d_u_r * dx_inv; if (ROWS & 8) J[17] += t.d_v_t * dx_inv;
+        if (ROWS & 8) J[18] += t.d_v_b * dx_inv; if (ROWS & 8) J[19] += t.d_eta_H * N.e_c * dx_inv;
     }
     {
         SigN s = sigma_xx(N.u_ll, u_l, N.v_lt, N.v_lb, N.e_l * N.H_l, dx_inv, j - 1, nx);
-        r[0] -= s.res * dx_inv;
-        J[0] -= s.d_u_r * dx_inv;
+        if (ROWS & 1) r[0] -= s.res * dx_inv;
+        if (ROWS & 1) J[0] -= s.d_u_r * dx_inv;
     }
     {
         SigN s = sigma_xx(u_r, N.u_rr, N.v_rt, N.v_rb, N.e_r * N.H_r, dx_inv, j + 1, nx);
-        r[1] += s.res * dx_inv;
-        J[6] += s.d_u_l * dx_inv;
+        if (ROWS & 2) r[1] += s.res * dx_inv;
+        if (ROWS & 2) J[6] += s.d_u_l * dx_inv;
     }
     {
         SigN s = sigma_yy(N.u_tl, N.u_tr, N.v_tt, v_t, N.e_t * N.H_t, dx_inv, i - 1, ny);
-        r[2] += s.res * dx_inv;
-        J[12] += s.d_v_b * dx_inv;
+        if (ROWS & 4) r[2] += s.res * dx_inv;
+        if (ROWS & 4) J[12] += s.d_v_b * dx_inv;
     }
     {
         SigN s = sigma_yy(N.u_bl, N.u_br, v_b, N.v_bb, N.e_b * N.H_b, dx_inv, i + 1, ny);
-        r[3] -= s.res * dx_inv;
-        J[18] -= s.d_v_t * dx_inv;
+        if (ROWS & 8) r[3] -= s.res * dx_inv;
+        if (ROWS & 8) J[18] -= s.d_v_t * dx_inv;
     }
     {   // top-left vertex (i, j)
         double eH = etaH_vertex(N.e_tl, N.e_t, N.e_l, N.e_c, N.H_tl, N.H_t, N.H_l, H_c);
         double dH = 0.25 * N.e_c;
         SigS s = sigma_xy(N.u_tl, u_l, N.v_lt, v_t, eH, dx_inv, i, j, ny, nx);
-        r[0] += s.res * dx_inv;
-        J[0] += s.d_u_b * dx_inv;
-        J[4] += s.d_eta_H * dH * dx_inv;
-        r[2] -= s.res * dx_inv;
-        J[12] -= s.d_v_r * dx_inv;
-        J[14] -= s.d_eta_H * dH * dx_inv;
+        if (ROWS & 1) r[0] += s.res * dx_inv;
+        if (ROWS & 1) J[0] += s.d_u_b * dx_inv;
+        if (ROWS & 1) J[4] += s.d_eta_H * dH * dx_inv;
+        if (ROWS & 4) r[2] -= s.res * dx_inv;
+        if (ROWS & 4) J[12] -= s.d_v_r * dx_inv;
+        if (ROWS & 4) J[14] -= s.d_eta_H * dH * dx_inv;
     }
     {   // top-right vertex (i, j+1)
         double eH = etaH_vertex(N.e_t, N.e_tr, N.e_c, N.e_r, N.H_t, N.H_tr, H_c, N.H_r);
         double dH = 0.25 * N.e_c;
         SigS s = sigma_xy(N.u_tr, u_r, v_t, N.v_rt, eH, dx_inv, i, j + 1, ny, nx);
-        r[1] += s.res * dx_inv;
-        J[6] += s.d_u_b * dx_inv;
-        J[9] += s.d_eta_H * dH * dx_inv;
-        r[2] += s.res * dx_inv;
-        J[12] += s.d_v_l * dx_inv;
-        J[14] += s.d_eta_H * dH * dx_inv;
+        if (ROWS & 2) r[1] += s.res * dx_inv;
+        if (ROWS & 2) J[6] += s.d_u_b * dx_inv;
+        if (ROWS & 2) J[9] += s.d_eta_H * dH * dx_inv;
+        if (ROWS & 4) r[2] += s.res * dx_inv;
+        if (ROWS & 4) J[12] += s.d_v_l * dx_inv;
+        if (ROWS & 4) J[14] += s.d_eta_H * dH * dx_inv;
     }
     {   // bottom-left vertex (i+1, j)
         double eH = etaH_vertex(N.e_l, N.e_c, N.e_bl, N.e_b, N.H_l, H_c, N.H_bl, N.H_b);
         double dH = 0.25 * N.e_c;
         SigS s = sigma_xy(u_l, N.u_bl, N.v_lb, v_b, eH, dx_inv, i + 1, j, ny, nx);
-        r[0] -= s.res * dx_inv;
-        J[0] -= s.d_u_t * dx_inv;
-        J[4] -= s.d_eta_H * dH * dx_inv;
-        r[3] -= s.res * dx_inv;
-        J[18] -= s.d_v_r * dx_inv;
-        J[19] -= s.d_eta_H * dH * dx_inv;
+        if (ROWS & 1) r[0] -= s.res * dx_inv;
+        if (ROWS & 1) J[0] -= s.d_u_t * dx_inv;
+        if (ROWS & 1) J[4] -= s.d_eta_H * dH * dx_inv;
+        if (ROWS & 8) r[3] -= s.res * dx_inv;
+        if (ROWS & 8) J[18] -= s.d_v_r * dx_inv;
+        if (ROWS & 8) J[19] -= s.d_eta_H * dH * dx_inv;
     }
     {   // bottom-right vertex (i+1, j+1)
         double eH = etaH_vertex(N.e_c, N.e_r, N.e_b, N.e_br, H_c, N.H_r, N.H_b, N.H_br);
         double dH = 0.25 * N.e_c;
         SigS s = sigma_xy(u_r, N.u_br, v_b, N.v_rb, eH, dx_inv, i + 1, j + 1, ny, nx);
-        r[1] -= s.res * dx_inv;
-        J[6] -= s.d_u_t * dx_inv;
-        J[9] -= s.d_eta_H * dH * dx_inv;
-        r[3] += s.res * dx_inv;
-        J[18] += s.d_v_l * dx_inv;
-        J[19] += s.d_eta_H * dH * dx_inv;
+        if (ROWS & 2) r[1] -= s.res * dx_inv;
+        if (ROWS & 2) J[6] -= s.d_u_t * dx_inv;
+        if (ROWS & 2) J[9] -= s.d_eta_H * dH * dx_inv;
+        if (ROWS & 8) r[3] += s.res * dx_inv;
+        if (ROWS & 8) J[18] += s.d_v_l * dx_inv;
+        if (ROWS & 8) J[19] += s.d_eta_H * dH * dx_inv;
     }
     {
         TauB t = tau_b_g(u_l, N.g_l.g, g_c.g, N.g_l.gd, g_c.gd, N.beta_l, N.beta_c, p);
-        r[0] += t.res; J[0] += t.d_u; J[4] += t.d_H_r;
+        if (ROWS & 1) r[0] += t.res; if (ROWS & 1) J[0] += t.d_u; if (ROWS & 1) J[4] += t.d_H_r;
     }
     {
         TauB t = tau_b_g(u_r, g_c.g, N.g_r.g, g_c.gd, N.g_r.gd, N.beta_c, N.beta_r, p);
-        r[1] += t.res; J[6] += t.d_u; J[9] += t.d_H_l;
+        if (ROWS & 2) r[1] += t.res; if (ROWS & 2) J[6] += t.d_u; if (ROWS & 2) J[9] += t.d_H_l;
     }
     {
         TauB t = tau_b_g(v_t, N.g_t.g, g_c.g, N.g_t.gd, g_c.gd, N.beta_t, N.beta_c, p);
-        r[2] += t.res; J[12] += t.d_u; J[14] += t.d_H_r;
+        if (ROWS & 4) r[2] += t.res; if (ROWS & 4) J[12] += t.d_u; if (ROWS & 4) J[14] += t.d_H_r;
     }
     {
         TauB t = tau_b_g(v_b, g_c.g, N.g_b.g, g_c.gd, N.g_b.gd, N.beta_c, N.beta_b, p);
-        r[3] += t.res; J[18] += t.d_u; J[19] += t.d_H_l;
+        if (ROWS & 8) r[3] += t.res; if (ROWS & 8) J[18] += t.d_u; if (ROWS & 8) J[19] += t.d_H_l;
     }
     {
         TauD t = tau_dx(N.H_l, H_c, N.bed_l, N.bed_c, dx_inv, j, nx);
-        r[0] -= t.res; J[4] -= t.d_H_r;
+        if (ROWS & 1) r[0] -= t.res; if (ROWS & 1) J[4] -= t.d_H_r;
     }
     {
         TauD t = tau_dx(H_c, N.H_r, N.bed_c, N.bed_r, dx_inv, j + 1, nx);
-        r[1] -= t.res; J[9] -= t.d_H_l;
+        if (ROWS & 2) r[1] -= t.res; if (ROWS & 2) J[9] -= t.d_H_l;
     }
     {
         TauD t = tau_dy(N.H_t, H_c, N.bed_t, N.bed_c, dx_inv, i, ny);
-        r[2] -= t.res; J[14] -= t.d_H_r;
+        if (ROWS & 4) r[2] -= t.res; if (ROWS & 4) J[14] -= t.d_H_r;
     }
     {
         TauD t = tau_dy(H_c, N.H_b, N.bed_c, N.bed_b, dx_inv, i + 1, ny);
-        r[3] -= t.res; J[19] -= t.d_H_l;
+        if (ROWS & 8) r[3] -= t.res; if (ROWS & 8) J[19] -= t.d_H_l;
     }
     if (MODE == 0) {
-        J[0] -= 1.0; J[6] -= 1.0; J[12] -= 1.0; J[18] -= 1.0; J[24] += 1.0;
+        if (ROWS & 1) J[0] -= 1.0; if (ROWS & 2) J[6] -= 1.0; if (ROWS & 4) J[12] -= 1.0; if (ROWS & 8) J[18] -= 1.0; if (ROWS & 16) J[24] += 1.0;
     }
 }
 

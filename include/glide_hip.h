@@ -257,6 +257,12 @@ int glide_batch_time_kernel(GlideBatch *batch, int kernel, int phase, int ty, in
  * not depend on the setting.  Process-wide; no reference counterpart. */
 int glide_set_persistence(int kernel, int wg_per_cu);
 
+/* Levels with at most max_cells cells run their Vanka sweeps row-split (four
+ * waves per 64 cells, each assembling part of the local block; same results,
+ * shorter per-wave instruction stream for latency-bound small levels); 0
+ * disables it.  Process-wide; no reference counterpart. */
+int glide_set_row_split(long max_cells);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *glide_last_error(void);
 
