@@ -216,6 +216,26 @@ class IcePhysics:
         g.mask.zero_()
         return float(torch.linalg.norm(g.r_adj))
 
+    # ------------------------------------------------------------ levels
+    def level_view(self, k: int) -> "IcePhysics":
+        """An IcePhysics over levels k.. of this hierarchy, sharing their
+        buffers (nothing is allocated): level k acts as the finest level of
+        forward, adjoint and set_parameters.  The multi-resolution inversion
+        solves on these views (examples/greenland/greenland_inverse.py:137-
+        142, where each level's Grid is used as the top of its own V-cycle)."""
+        if not 0 <= k < self.n_levels:
+            raise ValueError(f"level {k} outside 0..{self.n_levels - 1}")
+        if k == 0:
+            return self
+        view = object.__new__(IcePhysics)
+        view.__dict__.update(self.__dict__)
+        g = self.grids[k]
+        view.grid, view.grids = g, self.grids[k:]
+        view.ny, view.nx, view.dx, view.n_levels = g.ny, g.nx, g.dx, self.n_levels - k
+        view._desc = None
+        view.last_history = []
+        return view
+
     # ------------------------------------------------------------ misc API
     def reset_solution(self):
         """glide/physics.py:244-248"""
