@@ -47,7 +47,11 @@ class GlideSchedule(ctypes.Structure):
                 ("local_sweeps", c_int), ("n_newton", c_int), ("adj_sweeps", c_int), ("adj_coarse", c_int)]
 
 
-# name -> (restype, argtypes); the list mirrors include/glide_hip.h
+class GlideVtiArray(ctypes.Structure):
+    _fields_ = [("comp", c_void_p * 3), ("n_comp", c_int)]
+
+
+# name -> (restype, argtypes); the list mirrors include/glide_hip.h, glide_unet.h and glide_io.h
 SIGNATURES = {
     "glide_residual": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP, DP, DP, c_void_p]),
     "glide_jvp": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP, DP, c_void_p]),
@@ -116,6 +120,12 @@ SIGNATURES = {
                                             c_void_p]),
     "glide_set_persistence": (c_int, [c_int, c_int]),
     "glide_set_row_split": (c_int, [ctypes.c_long]),
+    "glide_vti_block_bytes": (ctypes.c_size_t, [POINTER(GlideVtiArray), c_int, c_int, c_int]),
+    "glide_vti_pack": (c_int, [POINTER(GlideVtiArray), c_int, c_int, c_int, c_int, c_void_p, ctypes.c_size_t,
+                               c_void_p]),
+    "glide_interp_grid": (c_int, [DP, c_int, c_int, DP, c_int, DP, c_int, DP, c_int, DP, c_int, c_double, DP,
+                                  c_void_p]),
+    "glide_bedmachine_thickness": (c_int, [DP, DP, ctypes.c_long, c_double, DP, c_void_p]),
     "glide_last_error": (c_char_p, []),
     "glide_version": (c_char_p, []),
 }

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libglide_hip.so")
-SOURCES = ["glide_hip.hip", "glide_batch.hip", "unet.hip"]
+SOURCES = ["glide_hip.hip", "glide_batch.hip", "unet.hip", "vti.hip", "grid_data.hip"]
 HEADERS = ["detmath.h", "ssa_device.h", "vanka_device.h", "tiles_device.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

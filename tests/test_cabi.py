@@ -11,7 +11,7 @@ from glide_amd import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-HEADERS = ("glide_hip.h", "glide_unet.h")
+HEADERS = ("glide_hip.h", "glide_unet.h", "glide_io.h")
 
 
 def declared():
