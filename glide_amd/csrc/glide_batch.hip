@@ -138,16 +138,18 @@ __global__ __launch_bounds__(BTX *PTY) void kb_apply(const BEntry *__restrict__ 
 }
 
 __global__ __launch_bounds__(256) void kb_activation(const BEntry *__restrict__ E, const int *__restrict__ pre, int n,
-                                                     const unsigned char *__restrict__ on)
+                                                     const unsigned char *__restrict__ on, int *__restrict__ any_active)
 {
     tile_loop(pre[n], [&](int t) {
         const int e = find_entry(pre, n, t);
         const GlideLevel &A = E[e].L;
         if (!on[E[e].member]) return;
         const int nbx = A.g.nx / RB + 1, lt = t - pre[e];
-        activation_block(A.g.ny, A.g.nx, A.r, A.U, A.gamma, A.active, lt % nbx, lt / nbx, nbx);
+        activation_block(A.g.ny, A.g.nx, A.r, A.U, A.gamma, A.active, lt % nbx, lt / nbx, nbx, any_active);
     });
 }
+
+__global__ void kb_set_int(int *__restrict__ x, int v) { *x = v; }
 
 // Element-wise passes on point tiles: point (i, j) owns u[i, j] (i < ny),
 // v[i, j] (j < nx) and the cell (i, j) (both).  Same expressions as the
@@ -337,6 +339,9 @@ struct GlideBatch {
     double *h_norm = nullptr;         // pinned
     unsigned char *h_on = nullptr;    // pinned
     BEntry *h_stage = nullptr;        // pinned upload staging
+    int *d_any = nullptr;             // any-block flags of two consecutive local passes
+    int *h_any = nullptr;             // pinned copies of d_any
+    hipEvent_t any_ev[2] = {};        // recorded after each copy
     std::vector<int> h_tables;
 };
 
@@ -470,12 +475,30 @@ int b_point(const GlideBatch *b, int p, double thklim, hipStream_t st)
     return bcheck("batched element-wise pass");
 }
 
-int b_activation(const GlideBatch *b, int p, hipStream_t st)
+int b_activation(const GlideBatch *b, int p, hipStream_t st, int *any_active = nullptr)
 {
     const int tid = table_id(0, TK_BLOCK, 0);
     hipLaunchKernelGGL(kb_activation, dim3(grid_of(b, b->ph[p].total[tid], 8)), dim3(16, 16), 0, st, entries(b, p),
-                       table(b, p, tid), b->ph[p].n, b->d_on);
+                       table(b, p, tid), b->ph[p].n, b->d_on, any_active);
     return bcheck("batched activation");
+}
+
+// Activation of local pass `pass` with its any-block flag copied to the host
+// (slot pass & 1); sets *stop when the host has seen an idle pass: pass 0's
+// flag is waited for, from pass 2 on the previous pass's (see the
+// single-domain V-cycle for the argument that every later pass is idle).
+int b_checked_activation(const GlideBatch *b, int p, int pass, hipStream_t st, bool *stop)
+{
+    const int k = pass & 1;
+    hipLaunchKernelGGL(kb_set_int, dim3(1), dim3(1), 0, st, b->d_any + k, 0);
+    BTRY(b_activation(b, p, st, b->d_any + k));
+    hipError_t e = hipMemcpyAsync(b->h_any + k, b->d_any + k, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(b->any_ev[k], st);
+    int look = pass == 0 ? 0 : (pass >= 2 ? (k ^ 1) : -1);
+    if (e == hipSuccess && look >= 0) e = hipEventSynchronize(b->any_ev[look]);
+    if (e != hipSuccess) return bfail((int)e, std::string("local pass check: ") + hipGetErrorString(e));
+    *stop = look >= 0 && b->h_any[look] == 0;
+    return GLIDE_OK;
 }
 
 // glide/solver.py:71-157 over every entry of phase d at once
@@ -500,9 +523,16 @@ int b_fascd(const GlideBatch *b, int d, double thklim, const GlideSchedule &sc, 
     BTRY(b_point<OP_FAS_CORRECT>(b, d, thklim, st));
     BTRY(b_point<OP_GAMMA>(b, d, thklim, st));
     BTRY(b_sweeps(b, d, false, sc.post, sc.n_newton, st));
+    const bool skip = glide::skip_idle_local();
     for (int pass = 0; pass < sc.local_passes; pass++) {
         BTRY(b_residual(b, d, false, P.shape[BK_RESIDUAL], P.occ[BK_RESIDUAL], st));
-        BTRY(b_activation(b, d, st));
+        if (skip) {
+            bool stop = false;
+            BTRY(b_checked_activation(b, d, pass, st, &stop));
+            if (stop) break;
+        } else {
+            BTRY(b_activation(b, d, st));
+        }
         BTRY(b_sweeps(b, d, true, sc.local_sweeps, sc.n_newton, st));
     }
     return b_point<OP_FILL_GAMMA>(b, d, thklim, st);
@@ -535,6 +565,10 @@ void destroy(GlideBatch *b)
     (void)hipHostFree(b->h_norm);
     (void)hipHostFree(b->h_on);
     (void)hipHostFree(b->h_stage);
+    (void)hipFree(b->d_any);
+    (void)hipHostFree(b->h_any);
+    for (int k = 0; k < 2; k++)
+        if (b->any_ev[k]) (void)hipEventDestroy(b->any_ev[k]);
     delete b;
 }
 
@@ -623,6 +657,9 @@ int glide_batch_create(const GlideLevel *const *members, const int *n_levels, in
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_norm, n_members * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_on, n_members, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_stage, std::max(eb, tb), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_any, 2 * sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_any, 2 * sizeof(int), hipHostMallocDefault);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&b->any_ev[k], hipEventDisableTiming);
     if (e == hipSuccess) {
         std::memcpy(b->h_stage, b->h_tables.data(), tb);
         e = hipMemcpy(b->d_tables, b->h_stage, tb, hipMemcpyHostToDevice);

@@ -62,6 +62,9 @@ inline hipStream_t S(void *s) { return (hipStream_t)s; }
 int g_wg_per_cu[2] = {64, 4};
 // Levels with at most this many cells use the row-split Vanka sweep.
 long g_row_split_max_cells = 128 * 128;
+// Stop a level's local passes at the first pass that activates no block.
+bool g_skip_idle_local = true;
+long g_local_passes_run = 0, g_local_passes_skipped = 0;
 
 int n_cu()
 {
@@ -91,6 +94,7 @@ inline unsigned blocks1d(size_t n, int bs = 256) { return (unsigned)((n + bs - 1
 
 // error channel shared with the batched entry points (glide_batch.hip)
 int glide::set_error(int code, const char *msg) { return fail(code, msg); }
+bool glide::skip_idle_local() { return g_skip_idle_local; }
 
 // ---------------------------------------------------------------------------
 // Residual r = F(U) - f (- sub)  (glide/cuda/residuals.cu:6-292)
@@ -287,10 +291,12 @@ __global__ __launch_bounds__(NT) void k_vanka_apply(int ny, int nx, double *__re
 __global__ __launch_bounds__(256) void k_local_activation(int ny, int nx, const double *__restrict__ r,
                                                           const double *__restrict__ U,
                                                           const double *__restrict__ gamma,
-                                                          unsigned char *__restrict__ blk)
+                                                          unsigned char *__restrict__ blk, int *__restrict__ any_active)
 {
-    activation_block(ny, nx, r, U, gamma, blk, blockIdx.x, blockIdx.y, gridDim.x);
+    activation_block(ny, nx, r, U, gamma, blk, blockIdx.x, blockIdx.y, gridDim.x, any_active);
 }
+
+__global__ void k_set_int(int *__restrict__ x, int v) { *x = v; }
 
 // ---------------------------------------------------------------------------
 // Transfer operators (glide/cuda/utility.cu)
@@ -967,11 +973,56 @@ int launch_sweeps(const GlideGeom &g, const Params &p, double *U, const double *
 }
 
 int launch_activation(const GlideGeom &g, const double *r, const double *U, const double *gamma, unsigned char *blk,
-                      hipStream_t st)
+                      hipStream_t st, int *any_active = nullptr)
 {
     hipLaunchKernelGGL(k_local_activation, dim3(nbx_of(g.nx), nby_of(g.ny)), dim3(16, 16), 0, st, g.ny, g.nx, r, U,
-                       gamma, blk);
+                       gamma, blk, any_active);
     return check_launch("local activation");
+}
+
+// Host side of the idle-pass check: two pinned slots for the activation
+// flags of consecutive passes and an event after each copy.  Created on
+// first use, once per host thread and device, and kept for the life of the
+// process.
+struct PassCheck {
+    int *slot = nullptr;
+    hipEvent_t ev[2] = {};
+};
+constexpr int MAX_DEVICES = 64;
+thread_local PassCheck t_pass_check[MAX_DEVICES];
+
+int pass_check(PassCheck **out)
+{
+    int dev = 0;
+    hipError_t ge = hipGetDevice(&dev);
+    if (ge != hipSuccess) return fail((int)ge, std::string("local pass check: ") + hipGetErrorString(ge));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(GLIDE_E_ARG, "local pass check: device index out of range");
+    PassCheck &c = t_pass_check[dev];
+    if (!c.slot) {
+        hipError_t e = hipHostMalloc((void **)&c.slot, 2 * sizeof(int), hipHostMallocDefault);
+        for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&c.ev[k], hipEventDisableTiming);
+        if (e != hipSuccess) return fail((int)e, std::string("local pass check: ") + hipGetErrorString(e));
+    }
+    *out = &c;
+    return GLIDE_OK;
+}
+
+// Enqueue the copy of a pass's activation flag into its pinned slot.
+int post_flag(PassCheck *c, int k, const int *flag_dev, hipStream_t st)
+{
+    hipError_t e = hipMemcpyAsync(&c->slot[k], flag_dev, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[k], st);
+    if (e != hipSuccess) return fail((int)e, std::string("local pass check: ") + hipGetErrorString(e));
+    return GLIDE_OK;
+}
+
+// Wait for slot k's copy and report whether that activation marked a block.
+int read_flag(PassCheck *c, int k, bool *any)
+{
+    hipError_t e = hipEventSynchronize(c->ev[k]);
+    if (e != hipSuccess) return fail((int)e, std::string("local pass check: ") + hipGetErrorString(e));
+    *any = c->slot[k] != 0;
+    return GLIDE_OK;
 }
 
 int restrict_state(const GlideLevel &F, const GlideLevel &C, const double *src, double *dst, double *dst2,
@@ -1057,9 +1108,44 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
     hipLaunchKernelGGL(k_gamma, dim3(blocks1d(nc)), dim3(256), 0, st, ny, nx, A.gamma, wH, A.chi, nullptr);
     TRY(check_launch("fas correct"));
     TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, nullptr, sc.post, sc.n_newton, A.scratch, st));
+    // A pass whose activation marks no block leaves U unchanged (its apply
+    // passes add +0.0), so every later pass of this V-cycle recomputes the
+    // same residual and activation and is idle too.  With g_skip_idle_local
+    // the activation also raises an any-block flag (pass k's in int slot
+    // k & 1 at the start of the Vanka scratch, which the sweeps rewrite only
+    // after the flag has been copied out), and the host stops the level's
+    // passes once a flag reads 0:
+    // - pass 0's flag is waited for before its sweeps are enqueued (levels
+    //   that start idle are the common case and skip everything);
+    // - from pass 2 on, the host waits for the previous pass's flag, which
+    //   the GPU has produced while this pass's residual and activation were
+    //   already queued, so an active level never drains the stream; an
+    //   idle pass found this way has had its (idle) sweeps enqueued.
+    // The only difference from running every pass is the sign of zero
+    // entries of U that idle passes would turn from -0.0 into +0.0.
+    PassCheck *chk = nullptr;
+    if (g_skip_idle_local && sc.local_passes > 0) TRY(pass_check(&chk));
+    int *flags = reinterpret_cast<int *>(A.scratch);
     for (int pass = 0; pass < sc.local_passes; pass++) {
         TRY(launch_residual(A.g, p, A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma, st));
-        TRY(launch_activation(A.g, A.r, A.U, A.gamma, A.active, st));
+        if (chk) {
+            const int k = pass & 1;
+            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, flags + k, 0);
+            TRY(launch_activation(A.g, A.r, A.U, A.gamma, A.active, st, flags + k));
+            TRY(post_flag(chk, k, flags + k, st));
+            bool any = true;
+            if (pass == 0)
+                TRY(read_flag(chk, 0, &any));
+            else if (pass >= 2)
+                TRY(read_flag(chk, k ^ 1, &any));   // pass - 1 (pass 0 was active)
+            if (!any) {
+                g_local_passes_skipped += sc.local_passes - pass;
+                break;
+            }
+        } else {
+            TRY(launch_activation(A.g, A.r, A.U, A.gamma, A.active, st));
+        }
+        g_local_passes_run++;
         TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, A.active, sc.local_sweeps, sc.n_newton,
                           A.scratch, st));
     }
@@ -1145,6 +1231,21 @@ int adjoint_vc(const GlideLevel *L, int n, int k, const GlideSchedule &sc, hipSt
 extern "C" {
 
 const char *glide_last_error(void) { return g_last_error.c_str(); }
+int glide_set_local_skip(int on)
+{
+    g_skip_idle_local = on != 0;
+    return GLIDE_OK;
+}
+
+int glide_local_pass_counts(long *run, long *skipped, int reset)
+{
+    if (!run || !skipped) return fail(GLIDE_E_NULL, "local_pass_counts: NULL argument");
+    *run = g_local_passes_run;
+    *skipped = g_local_passes_skipped;
+    if (reset) g_local_passes_run = g_local_passes_skipped = 0;
+    return GLIDE_OK;
+}
+
 int glide_set_row_split(long max_cells)
 {
     if (max_cells < 0) return fail(GLIDE_E_ARG, "max_cells must be >= 0");

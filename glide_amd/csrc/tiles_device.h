@@ -14,6 +14,8 @@ namespace glide {
 
 // records msg as glide_last_error() and returns code (host side)
 int set_error(int code, const char *msg);
+// whether V-cycles stop a level's local passes at the first idle pass
+bool skip_idle_local();
 
 constexpr int RB = 14;  // reference launch-block interior (glide/grid.py:184-186)
 constexpr int NXCD = 8; // XCDs per MI355X
@@ -483,7 +485,8 @@ __device__ __forceinline__ void apply_tile(int ny, int nx, double *__restrict__ 
 // reads its 16x16 footprint (glide/solver.py:148-154, glide/cuda/vanka.cu:870-888).
 __device__ __forceinline__ void activation_block(int ny, int nx, const double *__restrict__ r,
                                                  const double *__restrict__ U, const double *__restrict__ gamma,
-                                                 unsigned char *__restrict__ blk, int bx, int by, int nbx)
+                                                 unsigned char *__restrict__ blk, int bx, int by, int nbx,
+                                                 int *__restrict__ any_active = nullptr)
 {
     const int i = by * RB - 1 + (int)threadIdx.y, j = bx * RB - 1 + (int)threadIdx.x;
     const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
@@ -497,7 +500,10 @@ __device__ __forceinline__ void activation_block(int ny, int nx, const double *_
         bad = te > 0.01;
     }
     int any = __syncthreads_or(bad);
-    if (threadIdx.x == 0 && threadIdx.y == 0) blk[by * nbx + bx] = (unsigned char)(any != 0);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        blk[by * nbx + bx] = (unsigned char)(any != 0);
+        if (any && any_active) *any_active = 1;   // every writer stores the same value
+    }
 }
 
 // ---------------------------------------------------------------------------

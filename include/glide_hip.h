@@ -263,6 +263,24 @@ int glide_set_persistence(int kernel, int wg_per_cu);
  * disables it.  Process-wide; no reference counterpart. */
 int glide_set_row_split(long max_cells);
 
+/* on != 0: the FASCD V-cycle stops a level's local passes once a pass's
+ * activation marks no block (glide/solver.py:147-155 runs all ten; an idle
+ * pass changes nothing, and neither can any pass after it).  The host reads
+ * each pass's 4-byte any-block flag: it waits for pass 0's before enqueuing
+ * its sweeps, and for pass k-1's (already computed) when enqueuing pass k,
+ * so at most one idle pass per level is still run.  The V-cycle call then
+ * returns only after the GPU has reached its last local pass.  First use on
+ * a host thread creates 8 bytes of pinned host memory and two events, kept
+ * for the life of the process.  The batched V-cycle does the same over all
+ * enabled members (a pass is idle when no member's activation marks a
+ * block), with its flags owned by the batch plan.  Process-wide switch;
+ * default on. */
+int glide_set_local_skip(int on);
+
+/* Local passes run and skipped by the single-domain FASCD V-cycle since the
+ * last reset (process-wide counters; reset != 0 zeroes them after reading). */
+int glide_local_pass_counts(long *run, long *skipped, int reset);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *glide_last_error(void);
 
