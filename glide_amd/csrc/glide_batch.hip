@@ -109,7 +109,7 @@ __global__ __launch_bounds__(BTX *TY) void kb_residual(const BEntry *__restrict_
 }
 
 template <int TY, bool LOCAL>
-__global__ __launch_bounds__(BTX *TY) void kb_vanka(const BEntry *__restrict__ E, const int *__restrict__ pre, int n,
+__global__ __launch_bounds__(BTX *TY) __attribute__((amdgpu_waves_per_eu(2))) void kb_vanka(const BEntry *__restrict__ E, const int *__restrict__ pre, int n,
                                                      const unsigned char *__restrict__ on, int n_newton)
 {
     __shared__ double eta_s[TY + 2][BTX + 2];

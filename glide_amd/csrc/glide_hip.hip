@@ -118,8 +118,11 @@ __global__ __launch_bounds__(NT) void k_residual(GlideGeom g, Params p, double *
 // cd (5 x nc, structure of arrays) receives 0.5*(x - x0) for the four faces
 // and (H - H0) for the cell.
 // ---------------------------------------------------------------------------
+// two waves per SIMD: the register budget both instantiated cell bodies
+// (interior and general) must share
 template <bool LOCAL>
-__global__ __launch_bounds__(NT) void k_vanka(GlideGeom g, Params p, const double *__restrict__ U,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_vanka(GlideGeom g, Params p,
+                                                                                   const double *__restrict__ U,
                                               const double *__restrict__ f, const double *__restrict__ bed,
                                               const double *__restrict__ B, const double *__restrict__ beta,
                                               const double *__restrict__ gamma, const unsigned char *__restrict__ blk,

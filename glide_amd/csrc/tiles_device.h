@@ -400,6 +400,52 @@ __device__ __forceinline__ CellNbhd load_nbhd(int i, int j, int ny, int nx, size
 }
 
 // ---------------------------------------------------------------------------
+// One cell's Vanka update: neighbourhood, damped Newton iterations, and the
+// per-cell deltas (glide/cuda/vanka.cu:56-439).
+// INTERIOR: the cell lies in [1, ny-2] x [1, nx-2].  Every boundary test of
+// the assembly compares a cell index in [i-1, i+1], or a face or vertex index
+// in [i, i+1] (likewise for j), with 0 and with ny (nx); all of them take the
+// interior branch for such a cell, as they do for the cell (1, 1) of a 3 x 3
+// grid.  Passing those constants lets the compiler drop the tests; the
+// arithmetic, and so every bit of the result, is unchanged.
+// ---------------------------------------------------------------------------
+template <bool INTERIOR, class EtaFn>
+__device__ __forceinline__ void vanka_cell(const GlideGeom &g, const Params &p, const double *__restrict__ u,
+                                           const double *__restrict__ v, const double *__restrict__ H,
+                                           const double *__restrict__ f, const double *__restrict__ bed,
+                                           const double *__restrict__ beta, const double *__restrict__ gamma,
+                                           int n_newton, double *__restrict__ cd, int i, int j, const EtaFn &E)
+{
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+    CellNbhd N = load_nbhd(i, j, ny, nx, nu, nv, u, v, H, f, bed, beta, E);
+    nbhd_grounding(N, p);
+    const double u_l0 = ld_u(u, i, j, ny, nx), u_r0 = ld_u(u, i, j + 1, ny, nx);
+    const double v_t0 = ld_v(v, i, j, ny, nx), v_b0 = ld_v(v, i + 1, j, ny, nx);
+    const double H_c0 = ld_c(H, i, j, ny, nx), th = ld_c(gamma, i, j, ny, nx);
+    double u_l = u_l0, u_r = u_r0, v_t = v_t0, v_b = v_b0, H_c = H_c0;
+    const int ai = INTERIOR ? 1 : i, aj = INTERIOR ? 1 : j, any = INTERIOR ? 3 : ny, anx = INTERIOR ? 3 : nx;
+    for (int k = 0; k < n_newton; k++) {
+        double J[25], r[5], d[5];
+        vanka_assemble<0>(N, p, ai, aj, any, anx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, false, J, r);
+        lu5_solve(J, r, d);
+        u_l -= 0.25 * d[0];
+        u_r -= 0.25 * d[1];
+        v_t -= 0.25 * d[2];
+        v_b -= 0.25 * d[3];
+        H_c -= 0.25 * d[4];
+        H_c = fmax(H_c, th);
+    }
+    const size_t ci = (size_t)i * nx + j;
+    cd[ci] = 0.5 * (u_l - u_l0);
+    cd[nc + ci] = 0.5 * (u_r - u_r0);
+    cd[2 * nc + ci] = 0.5 * (v_t - v_t0);
+    cd[3 * nc + ci] = 0.5 * (v_b - v_b0);
+    cd[4 * nc + ci] = H_c - H_c0;
+}
+
+// ---------------------------------------------------------------------------
 // Forward Vanka: per-cell damped Newton (glide/cuda/vanka.cu:3-439, 844-1295)
 // cd (5 x nc, structure of arrays) receives 0.5*(x - x0) for the four faces
 // and (H - H0) for the cell.
@@ -426,31 +472,14 @@ __device__ __forceinline__ void vanka_tile(const GlideGeom &g, const Params &p, 
     __syncthreads();
     if (!mine) return;
     auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
-    const double dx_inv = 1.0 / g.dx, dt = g.dt;
-
-    CellNbhd N = load_nbhd(i, j, ny, nx, nu, nv, u, v, H, f, bed, beta, E);
-    nbhd_grounding(N, p);
-    const double u_l0 = ld_u(u, i, j, ny, nx), u_r0 = ld_u(u, i, j + 1, ny, nx);
-    const double v_t0 = ld_v(v, i, j, ny, nx), v_b0 = ld_v(v, i + 1, j, ny, nx);
-    const double H_c0 = ld_c(H, i, j, ny, nx), th = ld_c(gamma, i, j, ny, nx);
-    double u_l = u_l0, u_r = u_r0, v_t = v_t0, v_b = v_b0, H_c = H_c0;
-    for (int k = 0; k < n_newton; k++) {
-        double J[25], r[5], d[5];
-        vanka_assemble<0>(N, p, i, j, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, false, J, r);
-        lu5_solve(J, r, d);
-        u_l -= 0.25 * d[0];
-        u_r -= 0.25 * d[1];
-        v_t -= 0.25 * d[2];
-        v_b -= 0.25 * d[3];
-        H_c -= 0.25 * d[4];
-        H_c = fmax(H_c, th);
-    }
-    const size_t ci = (size_t)i * nx + j;
-    cd[ci] = 0.5 * (u_l - u_l0);
-    cd[nc + ci] = 0.5 * (u_r - u_r0);
-    cd[2 * nc + ci] = 0.5 * (v_t - v_t0);
-    cd[3 * nc + ci] = 0.5 * (v_b - v_b0);
-    cd[4 * nc + ci] = H_c - H_c0;
+    // A tile whose cells all lie in [1, ny-2] x [1, nx-2] runs the cell
+    // with its boundary tests resolved at compile time (same bits).  The
+    // whole per-cell body is instantiated twice so that no value is live
+    // across the branch.
+    if (i0 >= 1 && i0 + TY <= ny - 1 && j0 >= 1 && j0 + TX <= nx - 1)
+        vanka_cell<true>(g, p, u, v, H, f, bed, beta, gamma, n_newton, cd, i, j, E);
+    else
+        vanka_cell<false>(g, p, u, v, H, f, bed, beta, gamma, n_newton, cd, i, j, E);
 }
 
 // Gather the per-cell updates: U += 0.5 * delta (glide/grid.py:305-317)
