@@ -85,7 +85,7 @@ dim3 persistent_grid(long tiles, int per_cu)
     return dim3((unsigned)((g + NXCD - 1) / NXCD * NXCD));
 }
 dim3 vanka_grid(int ny, int nx) { return persistent_grid((long)cdiv(nx, TX) * cdiv(ny, TY), g_wg_per_cu[0]); }
-dim3 residual_grid(int ny, int nx) { return persistent_grid((long)cdiv(nx + 1, TX) * cdiv(ny + 1, TY), g_wg_per_cu[1]); }
+dim3 residual_grid(int ny, int nx) { return persistent_grid((long)cdiv(nx, TX) * cdiv(ny, TY), g_wg_per_cu[1]); }
 dim3 point_grid(int ny, int nx) { return dim3((nx + 1 + TX - 1) / TX, (ny + 1 + TY - 1) / TY); }
 dim3 cell_grid(int ny, int nx) { return dim3((nx + TX - 1) / TX, (ny + TY - 1) / TY); }
 inline unsigned blocks1d(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
@@ -106,10 +106,10 @@ __global__ __launch_bounds__(NT) void k_residual(GlideGeom g, Params p, double *
                                                  const double *__restrict__ mask, const double *__restrict__ gamma)
 {
     __shared__ ResidLDS<TX, TY> S;
-    const int tx = cdiv(g.nx + 1, TX), total = tx * cdiv(g.ny + 1, TY);
+    const int tx = cdiv(g.nx, TX), total = tx * cdiv(g.ny, TY);   // cell tiles; edge faces by the last tiles
     tile_loop(total, [&](int t) {
         __syncthreads();   // the previous tile's readers of S are done
-        residual_tile_lds<TX, TY>(g, p, r, U, f, sub, bed, B, beta, mask, gamma, t % tx, t / tx, S);
+        residual_tile_lds<TX, TY, true>(g, p, r, U, f, sub, bed, B, beta, mask, gamma, t % tx, t / tx, S);
     });
 }
 

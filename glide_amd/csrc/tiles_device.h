@@ -75,143 +75,13 @@ __device__ __forceinline__ void fill_eta_tile(double (*eta_s)[TX + 2], int i0, i
 }
 
 // ---------------------------------------------------------------------------
-// Residual r = F(U) - f (- sub)  (glide/cuda/residuals.cu:6-292)
-// ---------------------------------------------------------------------------
-template <int TX, int TY>
-__device__ __forceinline__ void residual_tile(const GlideGeom &g, const Params &p, double *__restrict__ r,
-                                              const double *__restrict__ U, const double *__restrict__ f,
-                                              const double *__restrict__ sub, const double *__restrict__ bed,
-                                              const double *__restrict__ B, const double *__restrict__ beta,
-                                              const double *__restrict__ mask, const double *__restrict__ gamma,
-                                              int bx, int by, double (*eta_s)[TX + 2])
-{
-    const int ny = g.ny, nx = g.nx;
-    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
-    const double *u = U, *v = U + nu, *H = U + nu + nv;
-    const int i0 = by * TY, j0 = bx * TX;
-    fill_eta_tile<TX, TY>(eta_s, i0, j0, u, v, B, p, g.dx, ny, nx);
-    __syncthreads();
-    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
-    if (i > ny || j > nx) return;
-    auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
-    const double dx_inv = 1.0 / g.dx, dt = g.dt;
-
-    if (i < ny && j < nx) {
-        double H_c = ld_c(H, i, j, ny, nx);
-        double rH = H_c / dt - (f ? ld_c(f + nu + nv, i, j, ny, nx) : 0.0);
-        Calv jc = calving(H_c, ld_c(bed, i, j, ny, nx), p);
-        rH -= jc.res;
-        double H_l = ld_c(H, i, j - 1, ny, nx);
-        Flux q = flux_x(ld_u(u, i, j, ny, nx), H_l, H_c, j, nx);
-        rH -= q.res * dx_inv;
-        double H_r = ld_c(H, i, j + 1, ny, nx);
-        q = flux_x(ld_u(u, i, j + 1, ny, nx), H_c, H_r, j + 1, nx);
-        rH += q.res * dx_inv;
-        double H_t = ld_c(H, i - 1, j, ny, nx);
-        q = flux_y(ld_v(v, i, j, ny, nx), H_t, H_c, i, ny);
-        rH += q.res * dx_inv;
-        double H_b = ld_c(H, i + 1, j, ny, nx);
-        q = flux_y(ld_v(v, i + 1, j, ny, nx), H_c, H_b, i + 1, ny);
-        rH -= q.res * dx_inv;
-        double m = ld_c(mask, i, j, ny, nx);
-        double th = ld_c(gamma, i, j, ny, nx);
-        double val = (1.0 - m) * rH + m * (H_c - th);
-        size_t k = nu + nv + (size_t)i * nx + j;
-        r[k] = sub ? val - sub[k] : val;
-    }
-    if (i < ny) {
-        double ru = f ? -ld_u(f, i, j, ny, nx) : -0.0;
-        {
-            double eH = E(i, j) * ld_c(H, i, j, ny, nx);
-            SigN s = sigma_xx(ld_u(u, i, j, ny, nx), ld_u(u, i, j + 1, ny, nx), ld_v(v, i, j, ny, nx),
-                              ld_v(v, i + 1, j, ny, nx), eH, dx_inv, j, nx);
-            ru += s.res * dx_inv;
-        }
-        {
-            double eH = E(i, j - 1) * ld_c(H, i, j - 1, ny, nx);
-            SigN s = sigma_xx(ld_u(u, i, j - 1, ny, nx), ld_u(u, i, j, ny, nx), ld_v(v, i, j - 1, ny, nx),
-                              ld_v(v, i + 1, j - 1, ny, nx), eH, dx_inv, j - 1, nx);
-            ru -= s.res * dx_inv;
-        }
-        {
-            double eH = etaH_vertex(E(i - 1, j - 1), E(i - 1, j), E(i, j - 1), E(i, j), ld_c(H, i - 1, j - 1, ny, nx),
-                                    ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx));
-            SigS s = sigma_xy(ld_u(u, i - 1, j, ny, nx), ld_u(u, i, j, ny, nx), ld_v(v, i, j - 1, ny, nx),
-                              ld_v(v, i, j, ny, nx), eH, dx_inv, i, j, ny, nx);
-            ru += s.res * dx_inv;
-        }
-        {
-            double eH = etaH_vertex(E(i, j - 1), E(i, j), E(i + 1, j - 1), E(i + 1, j), ld_c(H, i, j - 1, ny, nx),
-                                    ld_c(H, i, j, ny, nx), ld_c(H, i + 1, j - 1, ny, nx), ld_c(H, i + 1, j, ny, nx));
-            SigS s = sigma_xy(ld_u(u, i, j, ny, nx), ld_u(u, i + 1, j, ny, nx), ld_v(v, i + 1, j - 1, ny, nx),
-                              ld_v(v, i + 1, j, ny, nx), eH, dx_inv, i + 1, j, ny, nx);
-            ru -= s.res * dx_inv;
-        }
-        {
-            TauB t = tau_b(ld_u(u, i, j, ny, nx), ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx),
-                           ld_c(bed, i, j - 1, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i, j - 1, ny, nx),
-                           ld_c(beta, i, j, ny, nx), p);
-            ru += t.res;
-        }
-        {
-            TauD t = tau_dx(ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx), ld_c(bed, i, j - 1, ny, nx),
-                            ld_c(bed, i, j, ny, nx), dx_inv, j, nx);
-            ru -= t.res;
-        }
-        size_t k = (size_t)i * (nx + 1) + j;
-        r[k] = sub ? ru - sub[k] : ru;
-    }
-    if (j < nx) {
-        double rv = f ? -ld_v(f + nu, i, j, ny, nx) : -0.0;
-        {
-            double eH = E(i - 1, j) * ld_c(H, i - 1, j, ny, nx);
-            SigN s = sigma_yy(ld_u(u, i - 1, j, ny, nx), ld_u(u, i - 1, j + 1, ny, nx), ld_v(v, i - 1, j, ny, nx),
-                              ld_v(v, i, j, ny, nx), eH, dx_inv, i - 1, ny);
-            rv += s.res * dx_inv;
-        }
-        {
-            double eH = E(i, j) * ld_c(H, i, j, ny, nx);
-            SigN s = sigma_yy(ld_u(u, i, j, ny, nx), ld_u(u, i, j + 1, ny, nx), ld_v(v, i, j, ny, nx),
-                              ld_v(v, i + 1, j, ny, nx), eH, dx_inv, i, ny);
-            rv -= s.res * dx_inv;
-        }
-        {
-            double eH = etaH_vertex(E(i - 1, j - 1), E(i - 1, j), E(i, j - 1), E(i, j), ld_c(H, i - 1, j - 1, ny, nx),
-                                    ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j - 1, ny, nx), ld_c(H, i, j, ny, nx));
-            SigS s = sigma_xy(ld_u(u, i - 1, j, ny, nx), ld_u(u, i, j, ny, nx), ld_v(v, i, j - 1, ny, nx),
-                              ld_v(v, i, j, ny, nx), eH, dx_inv, i, j, ny, nx);
-            rv -= s.res * dx_inv;
-        }
-        {
-            double eH = etaH_vertex(E(i - 1, j), E(i - 1, j + 1), E(i, j), E(i, j + 1), ld_c(H, i - 1, j, ny, nx),
-                                    ld_c(H, i - 1, j + 1, ny, nx), ld_c(H, i, j, ny, nx), ld_c(H, i, j + 1, ny, nx));
-            SigS s = sigma_xy(ld_u(u, i - 1, j + 1, ny, nx), ld_u(u, i, j + 1, ny, nx), ld_v(v, i, j, ny, nx),
-                              ld_v(v, i, j + 1, ny, nx), eH, dx_inv, i, j + 1, ny, nx);
-            rv += s.res * dx_inv;
-        }
-        {
-            TauB t = tau_b(ld_v(v, i, j, ny, nx), ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j, ny, nx),
-                           ld_c(bed, i - 1, j, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i - 1, j, ny, nx),
-                           ld_c(beta, i, j, ny, nx), p);
-            rv += t.res;
-        }
-        {
-            TauD t = tau_dy(ld_c(H, i - 1, j, ny, nx), ld_c(H, i, j, ny, nx), ld_c(bed, i - 1, j, ny, nx),
-                            ld_c(bed, i, j, ny, nx), dx_inv, i, ny);
-            rv -= t.res;
-        }
-        size_t k = nu + (size_t)i * nx + j;
-        r[k] = sub ? rv - sub[k] : rv;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Residual, LDS-staged form: the tile's u, v (two-ring halo), H, bed, beta
-// (one-ring halo) are staged once; every cell's viscosity, grounding sigmoid,
-// sigma_xx and sigma_yy and every vertex's sigma_xy are evaluated once into
-// LDS; each point then sums its rows from LDS.  Every quantity is the same
-// expression of the same (clamped) values as in residual_tile, and each row
-// sums its terms in the same order, so the two give identical bits.
+// Residual r = F(U) - f (- sub)  (glide/cuda/residuals.cu:6-292), LDS-staged:
+// the tile's u, v (two-ring halo), H, bed, beta (one-ring halo) are staged
+// once; every cell's viscosity, grounding sigmoid, sigma_xx and sigma_yy and
+// every vertex's sigma_xy are evaluated once into LDS; each point then sums
+// its rows from LDS.  Every quantity is the same expression of the same
+// (clamped) values as in the per-point form of the reference and the oracle,
+// and each row sums its terms in the same order, so the bits are the same.
 // ---------------------------------------------------------------------------
 template <int TX, int TY>
 struct ResidLDS {
@@ -228,7 +98,12 @@ struct ResidLDS {
     double sxy[TY + 1][TX + 1]; // vertices [i0+a][j0+b]
 };
 
-template <int TX, int TY>
+// EDGE_FACES: the tiles cover the cells only (cdiv(nx, TX) x cdiv(ny, TY));
+// where a tile ends exactly at the last column (row), its last lane (row of
+// threads) also computes the u faces at j = nx (v faces at i = ny), so the
+// extra face column and row need no tiles of their own.  Without it the
+// tiles cover the points [0, ny] x [0, nx].  Same arithmetic per point.
+template <int TX, int TY, bool EDGE_FACES = false>
 __device__ __forceinline__ void residual_tile_lds(const GlideGeom &g, const Params &p, double *__restrict__ r,
                                                   const double *__restrict__ U, const double *__restrict__ f,
                                                   const double *__restrict__ sub, const double *__restrict__ bed,
@@ -277,10 +152,12 @@ __device__ __forceinline__ void residual_tile_lds(const GlideGeom &g, const Para
         const int a = k / (TX + 2), b = k - a * (TX + 2);
         const int ci = i0 - 1 + a, cj = j0 - 1 + b;
         double sx = 0.0, sy = 0.0;
-        if (a >= 1 && a <= TY && b <= TX)   // cells [i0, i0+TY) x [j0-1, j0+TX)
+        // cells [i0, i0+TY) x [j0-1, j0+TX), and j0+TX for the faces at j = nx
+        if (a >= 1 && a <= TY && b <= TX + (EDGE_FACES ? 1 : 0))
             sx = sigma_xx(UL(ci, cj), UL(ci, cj + 1), VL(ci, cj), VL(ci + 1, cj), S.E[a][b] * S.H[a][b], dx_inv, cj,
                           nx).res;
-        if (a <= TY && b >= 1 && b <= TX)   // cells [i0-1, i0+TY) x [j0, j0+TX)
+        // cells [i0-1, i0+TY) x [j0, j0+TX), and i0+TY for the faces at i = ny
+        if (a <= TY + (EDGE_FACES ? 1 : 0) && b >= 1 && b <= TX)
             sy = sigma_yy(UL(ci, cj), UL(ci, cj + 1), VL(ci, cj), VL(ci + 1, cj), S.E[a][b] * S.H[a][b], dx_inv, ci,
                           ny).res;
         S.sxx[a][b] = sx;
@@ -294,16 +171,16 @@ __device__ __forceinline__ void residual_tile_lds(const GlideGeom &g, const Para
         S.sxy[a][b] = sigma_xy(UL(vi - 1, vj), UL(vi, vj), VL(vi, vj - 1), VL(vi, vj), eH, dx_inv, vi, vj, ny, nx).res;
     }
     __syncthreads();
-    // 4. rows of this thread's point (i, j)
-    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
-    if (i > ny || j > nx) return;
-    const int a = threadIdx.y + 1, b = threadIdx.x + 1;   // cell (i, j) in the one-ring arrays
+    // 4. rows of point (i, j): cell (i < ny, j < nx), u face (i < ny), v face (j < nx)
     auto SXX = [&](int ci, int cj) { return S.sxx[ci - i0 + 1][cj - j0 + 1]; };
     auto SYY = [&](int ci, int cj) { return S.syy[ci - i0 + 1][cj - j0 + 1]; };
     auto SXY = [&](int vi, int vj) { return S.sxy[vi - i0][vj - j0]; };
+    auto rows = [&](int i, int j) {
+    const int a = i - i0 + 1, b = j - j0 + 1;   // cell (i, j) in the one-ring arrays
     if (i < ny && j < nx) {
         double H_c = S.H[a][b];
-        double rH = H_c / dt - (f ? f[nu + nv + (size_t)i * nx + j] : 0.0);
+        const size_t kc = (size_t)i * nx + j;
+        double rH = H_c / dt - (f ? f[nu + nv + kc] : 0.0);
         Calv jc = calving_g(H_c, Ground{S.g[a][b], S.gd[a][b]}, p);
         rH -= jc.res;
         double H_l = S.H[a][b - 1];
@@ -318,7 +195,6 @@ __device__ __forceinline__ void residual_tile_lds(const GlideGeom &g, const Para
         double H_b = S.H[a + 1][b];
         q = flux_y(VL(i + 1, j), H_c, H_b, i + 1, ny);
         rH -= q.res * dx_inv;
-        const size_t kc = (size_t)i * nx + j;
         double m = mask[kc];
         double th = gamma[kc];
         double val = (1.0 - m) * rH + m * (H_c - th);
@@ -360,6 +236,14 @@ __device__ __forceinline__ void residual_tile_lds(const GlideGeom &g, const Para
             rv -= t.res;
         }
         r[k] = sub ? rv - sub[k] : rv;
+    }
+    };
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    if (i > ny || j > nx) return;
+    rows(i, j);
+    if (EDGE_FACES) {
+        if (j == nx - 1 && j0 + TX == nx && i < ny) rows(i, nx);   // u face at j = nx
+        if (i == ny - 1 && i0 + TY == ny && j < nx) rows(ny, j);   // v face at i = ny
     }
 }
 

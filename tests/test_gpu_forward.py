@@ -65,7 +65,7 @@ def rel(a, b):
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
 
 
-@pytest.mark.parametrize("shape", [(8, 10), (37, 45), (64, 129)])
+@pytest.mark.parametrize("shape", [(8, 10), (37, 45), (64, 129), (12, 64), (33, 128)])
 @pytest.mark.parametrize("gl", [False, True])
 def test_residual_bitwise(shape, gl):
     ny, nx = shape
