@@ -97,14 +97,14 @@ __global__ __launch_bounds__(BTX *TY) void kb_residual(const BEntry *__restrict_
         const BEntry &B = E[e];
         if (!on[B.member]) return;
         const GlideLevel &A = CHILD ? B.C : B.L;
-        const int lt = t - pre[e], tx = cdiv(A.g.nx + 1, BTX);
+        const int lt = t - pre[e], tx = cdiv(A.g.nx, BTX);   // cell tiles; edge faces by the last tiles
         __syncthreads();   // the previous tile's readers of S are done
         if (CHILD)
-            residual_tile_lds<BTX, TY>(A.g, to_params(A.p), A.f, A.U, nullptr, A.r, A.bed, A.B, A.beta, A.mask,
-                                       A.gamma, lt % tx, lt / tx, S);
+            residual_tile_lds<BTX, TY, true>(A.g, to_params(A.p), A.f, A.U, nullptr, A.r, A.bed, A.B, A.beta, A.mask,
+                                             A.gamma, lt % tx, lt / tx, S);
         else
-            residual_tile_lds<BTX, TY>(A.g, to_params(A.p), A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask,
-                                       A.gamma, lt % tx, lt / tx, S);
+            residual_tile_lds<BTX, TY, true>(A.g, to_params(A.p), A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask,
+                                             A.gamma, lt % tx, lt / tx, S);
     });
 }
 
@@ -389,7 +389,7 @@ int default_occ(int kernel, int ty)
 template <int TY>
 void launch_residual_ty(const GlideBatch *b, int p, bool child, unsigned grid, hipStream_t st)
 {
-    const int tid = table_id(child ? 1 : 0, TK_POINT, __builtin_ctz(TY));
+    const int tid = table_id(child ? 1 : 0, TK_CELL, __builtin_ctz(TY));
     if (child)
         hipLaunchKernelGGL((kb_residual<TY, true>), dim3(grid), dim3(BTX, TY), 0, st, entries(b, p), table(b, p, tid),
                            b->ph[p].n, b->d_on);
@@ -402,7 +402,7 @@ int b_residual(const GlideBatch *b, int p, bool child, int shape, int occ, hipSt
 {
     const BPhase &P = b->ph[p];
     const int kern = child ? BK_RESIDUAL_CHILD : BK_RESIDUAL;
-    const int total = P.total[table_id(child ? 1 : 0, TK_POINT, shape)];
+    const int total = P.total[table_id(child ? 1 : 0, TK_CELL, shape)];
     if (total == 0) return GLIDE_OK;
     const unsigned grid = grid_of(b, total, occ > 0 ? occ : default_occ(kern, ty_of(shape)));
     switch (shape) {
