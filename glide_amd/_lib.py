@@ -121,6 +121,7 @@ SIGNATURES = {
     "glide_set_persistence": (c_int, [c_int, c_int]),
     "glide_set_row_split": (c_int, [ctypes.c_long]),
     "glide_set_local_skip": (c_int, [c_int]),
+    "glide_count_nonfinite": (c_int, [DP, ctypes.c_long, c_void_p, POINTER(ctypes.c_ulonglong), c_void_p]),
     "glide_local_pass_counts": (c_int, [POINTER(ctypes.c_long), POINTER(ctypes.c_long), c_int]),
     "glide_vti_block_bytes": (ctypes.c_size_t, [POINTER(GlideVtiArray), c_int, c_int, c_int]),
     "glide_vti_pack": (c_int, [POINTER(GlideVtiArray), c_int, c_int, c_int, c_int, c_void_p, ctypes.c_size_t,

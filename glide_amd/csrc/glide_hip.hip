@@ -453,6 +453,20 @@ __global__ void k_forcing(size_t nc, double *__restrict__ fH, const double *__re
 }
 
 // ||(r_u, r_v, FB(r_H, H - gamma))||^2: per-block partial sums (fixed tree order)
+// Count of non-finite entries of x (NaN or +-Inf) into *count.
+__global__ __launch_bounds__(256) void k_count_nonfinite(const double *__restrict__ x, size_t n,
+                                                         unsigned long long *__restrict__ count)
+{
+    unsigned c = 0;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+        const double v = x[p];
+        c += !(v - v == 0.0);   // NaN - NaN and Inf - Inf are NaN
+    }
+    if (c) atomicAdd(count, (unsigned long long)c);
+}
+
+__global__ void k_zero_u64(unsigned long long *__restrict__ x) { *x = 0ull; }
+
 __global__ __launch_bounds__(256) void k_norm_partial(int ny, int nx, const double *__restrict__ r,
                                                       const double *__restrict__ U,
                                                       const double *__restrict__ gamma, double *__restrict__ part)
@@ -1479,6 +1493,23 @@ int glide_forward_setup(GlideLevel *L, int n, double dt, void *stream)
     double *fH = L[0].f + n_u(L[0].g.ny, L[0].g.nx) + n_v(L[0].g.ny, L[0].g.nx);
     hipLaunchKernelGGL(k_forcing, dim3(blocks1d(nc)), dim3(256), 0, S(stream), nc, fH, L[0].H_prev, L[0].smb, dt);
     return check_launch("forcing");
+}
+
+int glide_count_nonfinite(const double *x, long n, unsigned long long *counter, unsigned long long *count_host,
+                          void *stream)
+{
+    if (!x || !counter || !count_host) return fail(GLIDE_E_NULL, "count_nonfinite: NULL argument");
+    if (n < 0) return fail(GLIDE_E_SIZE, "count_nonfinite: negative size");
+    hipStream_t st = S(stream);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, counter);
+    if (n > 0)
+        hipLaunchKernelGGL(k_count_nonfinite, dim3((unsigned)std::min<size_t>(((size_t)n + 255) / 256, 2048)), dim3(256),
+                           0, st, x, (size_t)n, counter);
+    TRY(check_launch("count_nonfinite"));
+    hipError_t e = hipMemcpyAsync(count_host, counter, sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail((int)e, std::string("count_nonfinite: ") + hipGetErrorString(e));
+    return GLIDE_OK;
 }
 
 int glide_residual_norm(const GlideLevel *level, double *out_host, void *stream)

@@ -8,6 +8,9 @@ views of the state.
 from __future__ import annotations
 
 import ctypes
+import json
+import math
+import time
 
 import numpy as np
 import torch
@@ -52,6 +55,8 @@ class IcePhysics:
         self.schedule = None          # None = the reference's sweep counts
         self._desc = None
         self.last_history: list[float] = []
+        self.steps = 0                # forward steps taken (numbers the log records)
+        self._nonfinite = torch.zeros(1, dtype=torch.int64, device=self.device)
 
     # ------------------------------------------------------------ helpers
     def _levels(self):
@@ -70,28 +75,47 @@ class IcePhysics:
                                            adj_coarse)
 
     # ------------------------------------------------------------ inputs
+    def _check_finite(self, name: str, t: torch.Tensor):
+        """Raise ValueError when t holds NaN or +-Inf (counted on the GPU)."""
+        n = ctypes.c_ulonglong(0)
+        t = t.contiguous()
+        _lib.call("glide_count_nonfinite", t.data_ptr(), t.numel(), ctypes.c_void_p(self._nonfinite.data_ptr()),
+                  ctypes.byref(n), self._stream())
+        if n.value:
+            raise ValueError(f"{name} has {n.value} non-finite value(s)")
+
     def set_geometry(self, bed, thickness):
-        """glide/physics.py:82-99"""
+        """glide/physics.py:82-99; non-finite bed or thickness is rejected
+        (ValueError) before the hierarchy is touched."""
         g = self.grid
-        g.bed.copy_(_as_f64(bed, self.device).reshape(g.ny, g.nx))
-        g.H.copy_(_as_f64(thickness, self.device).reshape(g.ny, g.nx))
+        bed_t = _as_f64(bed, self.device).reshape(g.ny, g.nx)
+        H_t = _as_f64(thickness, self.device).reshape(g.ny, g.nx)
+        self._check_finite("bed", bed_t)
+        self._check_finite("thickness", H_t)
+        g.bed.copy_(bed_t)
+        g.H.copy_(H_t)
         g.H_prev.copy_(g.H)
         g.gamma.fill_(self.thklim)
         _lib.call("glide_propagate_geometry", self._levels(), self.n_levels, self.thklim, self._stream())
 
     def set_parameters(self, B=None, beta=None, smb=None):
-        """glide/physics.py:101-129"""
+        """glide/physics.py:101-129; non-finite values are rejected
+        (ValueError) before any of them is stored."""
         g = self.grid
-        if B is not None:
-            Bt = _as_f64(B, self.device)
+        Bt = _as_f64(B, self.device) if B is not None else None
+        bt = _as_f64(beta, self.device).reshape(g.ny, g.nx) if beta is not None else None
+        st = _as_f64(smb, self.device) if smb is not None else None
+        for name, t in (("B", Bt), ("beta", bt), ("smb", st)):
+            if t is not None:
+                self._check_finite(name, t.reshape(-1))
+        if Bt is not None:
             if Bt.ndim == 0:
                 g.B.fill_(float(Bt))
             else:
                 g.B.copy_(Bt.reshape(g.ny, g.nx))
-        if beta is not None:
-            g.beta.copy_(_as_f64(beta, self.device).reshape(g.ny, g.nx))
-        if smb is not None:
-            st = _as_f64(smb, self.device)
+        if bt is not None:
+            g.beta.copy_(bt)
+        if st is not None:
             if st.ndim == 0:
                 g.smb.fill_(float(st))
             else:
@@ -108,14 +132,19 @@ class IcePhysics:
         sched = ctypes.byref(self.schedule) if self.schedule is not None else None
         _lib.call("glide_fascd_vcycle", self._levels(), self.n_levels, self.thklim, sched, self._stream())
 
-    def forward(self, dt, n_vcycles=3, verbose=False, update_geometry=True, tol=None, max_vcycles=None):
+    def forward(self, dt, n_vcycles=3, verbose=False, update_geometry=True, tol=None, max_vcycles=None, log=None):
         """One implicit time step (glide/physics.py:141-209).
 
         Runs ``n_vcycles`` FASCD V-cycles; with ``tol`` it instead runs until
         the relative residual ||(r_u, r_v, FB(r_H, H - gamma))|| / r0 drops
         to ``tol`` (at most ``max_vcycles``).  The relative residual after
-        each cycle is kept in ``last_history`` when it was computed.
+        each cycle is kept in ``last_history`` when it was computed; a
+        non-finite one raises FloatingPointError.  ``log`` (a path or a
+        callable) receives one JSON record per step: step, dt, V-cycles,
+        the residual history and the wall time (the step is synchronised
+        for it).
         """
+        t0 = time.perf_counter() if log is not None else 0.0
         for g in self.grids:
             g.dt = float(dt)
         levels = self._levels()
@@ -123,20 +152,37 @@ class IcePhysics:
         track = verbose or tol is not None
         self.last_history = []
         r0 = self.residual_norm() if track else 0.0
+        if track and not math.isfinite(r0):
+            raise FloatingPointError(f"initial residual is {r0}")
         if verbose:
             print(f"  Initial: |r| = {r0:.2e}")
         n_max = n_vcycles if tol is None else (max_vcycles or max(n_vcycles, 50))
+        done = 0
         for it in range(n_max):
             self.vcycle()
+            done = it + 1
             if track:
                 rel = self.residual_norm() / r0 if r0 > 0 else 0.0
                 self.last_history.append(rel)
+                if not math.isfinite(rel):
+                    raise FloatingPointError(f"residual became {rel} in V-cycle {it}")
                 if verbose:
                     print(f"  V-cycle {it}: |r|/|r0| = {rel:.2e}")
                 if tol is not None and rel <= tol:
                     break
         if update_geometry:
             self.grid.H_prev.copy_(self.grid.H)
+        self.steps += 1
+        if log is not None:
+            torch.cuda.synchronize(self.device)
+            rec = {"step": self.steps, "dt": float(dt), "vcycles": done, "r0": r0 if track else None,
+                   "rel_residual": list(self.last_history), "ms": (time.perf_counter() - t0) * 1e3,
+                   "grid": [self.ny, self.nx], "n_levels": self.n_levels}
+            if callable(log):
+                log(rec)
+            else:
+                with open(log, "a") as fh:
+                    fh.write(json.dumps(rec) + "\n")
         return self.grid.u, self.grid.v, self.grid.H
 
     # ------------------------------------------------------------ adjoint
@@ -215,6 +261,57 @@ class IcePhysics:
         _lib.call("glide_adjoint_residual", ctypes.byref(lev[0]), self._stream())
         g.mask.zero_()
         return float(torch.linalg.norm(g.r_adj))
+
+    # ------------------------------------------------------------ checkpoint
+    _STATE = ("u", "v", "H", "H_prev", "bed", "B", "beta", "smb")
+
+    def state_dict(self) -> dict:
+        """Everything a later process needs to continue the time stepping
+        bit for bit: the finest level's state (u, v, H), H_prev, the inputs
+        (bed, B, beta, smb), the step count and the constructor arguments.
+        Tensors are copies on the host."""
+        g = self.grid
+        sd = {k: getattr(g, k).detach().to("cpu", copy=True) for k in self._STATE}
+        sd["config"] = {"ny": self.ny, "nx": self.nx, "dx": self.dx, "n_levels": self.n_levels, "n": self.n,
+                        "eps_reg": self.eps_reg, "thklim": self.thklim, "water_drag": self.water_drag,
+                        "calving_rate": self.calving_rate, "gl_sigmoid_c": self.gl_sigmoid_c,
+                        "gl_derivatives": self.gl_derivatives, "steps": self.steps}
+        return sd
+
+    def load_state_dict(self, sd: dict):
+        """Restore a state_dict into this hierarchy (same grid and physics
+        parameters): the inputs are set and restricted as set_geometry and
+        set_parameters do, then H and the velocities are restored."""
+        c = sd["config"]
+        mine = self.state_dict()["config"]
+        for k in ("ny", "nx", "dx", "n_levels", "n", "eps_reg", "thklim", "water_drag", "calving_rate",
+                  "gl_sigmoid_c", "gl_derivatives"):
+            if c[k] != mine[k]:
+                raise ValueError(f"checkpoint {k} = {c[k]} does not match this IcePhysics ({mine[k]})")
+        g = self.grid
+        self.set_geometry(sd["bed"], sd["H_prev"])
+        self.set_parameters(B=sd["B"], beta=sd["beta"], smb=sd["smb"])
+        for k in ("u", "v", "H"):
+            t = _as_f64(sd[k], self.device)
+            self._check_finite(k, t)
+            getattr(g, k).copy_(t.reshape(getattr(g, k).shape))
+        self.steps = int(c["steps"])
+
+    def save(self, path):
+        """torch.save of state_dict()."""
+        torch.save(self.state_dict(), path)
+
+    @classmethod
+    def load(cls, path, device=None) -> "IcePhysics":
+        """A new IcePhysics continued from a file written by save()
+        (loaded with weights_only=True: plain tensors and numbers only)."""
+        sd = torch.load(path, map_location="cpu", weights_only=True)
+        c = sd["config"]
+        ph = cls(c["ny"], c["nx"], c["dx"], n_levels=c["n_levels"], n=c["n"], eps_reg=c["eps_reg"],
+                 thklim=c["thklim"], water_drag=c["water_drag"], calving_rate=c["calving_rate"],
+                 gl_sigmoid_c=c["gl_sigmoid_c"], gl_derivatives=c["gl_derivatives"], device=device)
+        ph.load_state_dict(sd)
+        return ph
 
     # ------------------------------------------------------------ levels
     def level_view(self, k: int) -> "IcePhysics":

@@ -10,10 +10,15 @@
  *
  * Conventions for every entry point:
  *   - plain device pointers and sizes, no framework types;
- *   - nothing is allocated: scratch buffers are caller-provided;
+ *   - no device memory is allocated: scratch buffers are caller-provided
+ *     (batch plans own theirs; the V-cycle's idle-pass check keeps 8 bytes of
+ *     pinned host memory per host thread and device, see
+ *     glide_set_local_skip);
  *   - work is enqueued on the caller's stream (hipStream_t passed as void*,
- *     NULL = the default stream); no entry point synchronises the host
- *     except glide_residual_norm, which reads back one scalar;
+ *     NULL = the default stream); the entry points that read results back to
+ *     the host (glide_residual_norm, glide_count_nonfinite,
+ *     glide_batch_residual_norms, and the FASCD V-cycles while the idle-pass
+ *     check is on) synchronise with it;
  *   - return 0 on success, a negative GLIDE_E_* code for invalid arguments,
  *     or a positive hipError_t from the runtime; glide_last_error() gives a
  *     message for the last failure on the calling thread.
@@ -180,6 +185,13 @@ int glide_forward_setup(GlideLevel *levels, int n_levels, double dt, void *strea
 /* ||(r_u, r_v, FB(r_H, H - gamma))||_2 of the finest level into *out_host
  * (the verbose norm of glide/physics.py:176-203); synchronises the stream. */
 int glide_residual_norm(const GlideLevel *level, double *out_host, void *stream);
+
+/* Number of NaN / +-Inf entries among x[0..n) into *count_host, using the
+ * caller's 8-byte device counter; synchronises the stream.  No reference
+ * counterpart (the reference never checks its inputs): IcePhysics uses it to
+ * reject non-finite geometry and parameters. */
+int glide_count_nonfinite(const double *x, long n, unsigned long long *counter, unsigned long long *count_host,
+                          void *stream);
 
 /* One adjoint V-cycle (glide/solver.py:160-220). */
 int glide_adjoint_vcycle(const GlideLevel *levels, int n_levels, const GlideSchedule *sched,

@@ -53,6 +53,13 @@ def test_errors_are_reported_without_a_gpu():
     with pytest.raises(_lib.GlideError) as e:
         _lib.call("glide_residual", ctypes.byref(g), ctypes.byref(p), 1, 1, None, None, 1, 1, 1, 1, 1, None)
     assert e.value.code == _lib.GLIDE_E_ARG
+    n = ctypes.c_ulonglong()
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_count_nonfinite", None, 4, None, ctypes.byref(n), None)
+    assert e.value.code == _lib.GLIDE_E_NULL
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_count_nonfinite", 8, -1, 8, ctypes.byref(n), None)
+    assert e.value.code == _lib.GLIDE_E_SIZE
     assert _lib.load().glide_version().startswith(b"glide_amd")
 
 
