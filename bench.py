@@ -60,8 +60,8 @@ def parse():
 # 2048 x FETCH_SIZE) and a 256 MiB fill writes 262,144 KiB of WRITE_SIZE
 # (bytes = 1024 x WRITE_SIZE).  Vanka sweep + apply pass: FETCH 13,166 + 8,713.5,
 # WRITE 10,240 + 6,248; residual: FETCH 16,071.6, WRITE 6,175.
-TRAFFIC_512 = {"vanka": (2 * (13166.0 + 8713.5) + 10240.0 + 6248.0) * 1024,
-               "residual": (2 * 16071.6 + 6175.0) * 1024}
+TRAFFIC_512 = {"vanka": (2 * (13245.8 + 8713.5) + 10240.0 + 6248.0) * 1024,
+               "residual": (2 * 14146.4 + 6248.0) * 1024}
 
 
 def kernel_rooflines(ph, reps=20):
