@@ -34,8 +34,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from glide_amd.roofline import (FLOPS_NEWTON_ITER, PEAK_FP64_TFLOPS, PEAK_HBM_GBS,  # noqa: E402
-                                residual_bytes, sweep_flops)
+from glide_amd.roofline import (FLOPS_NEWTON_ITER, PEAK_BF16_TFLOPS, PEAK_FP64_TFLOPS,  # noqa: E402
+                                PEAK_HBM_GBS, residual_bytes, sweep_flops)
 
 
 def parse():
@@ -51,6 +51,8 @@ def parse():
                     help="only the cap_512 3-V-cycle steps (for profiling the reference schedule)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the kernel microbenchmarks (for profiling them alone)")
+    ap.add_argument("--cpu-report", action="store_true",
+                    help="only the CPU oracle: ms per V-cycle and per step of configs 0 and 1, one thread and all")
     return ap.parse_args()
 
 
@@ -58,8 +60,8 @@ def parse():
 # WRITE_SIZE (one run each) and calibrated in the same program on kernels of
 # known traffic: a 256 MiB torch.sum reads 131,134 KiB of FETCH_SIZE (bytes =
 # 2048 x FETCH_SIZE) and a 256 MiB fill writes 262,144 KiB of WRITE_SIZE
-# (bytes = 1024 x WRITE_SIZE).  Vanka sweep + apply pass: FETCH 13,166 + 8,713.5,
-# WRITE 10,240 + 6,248; residual: FETCH 16,071.6, WRITE 6,175.
+# (bytes = 1024 x WRITE_SIZE).  Vanka sweep + apply pass: FETCH 13,245.8 + 8,713.5,
+# WRITE 10,240 + 6,248; residual (cell tiles): FETCH 14,146.4, WRITE 6,248.
 TRAFFIC_512 = {"vanka": (2 * (13245.8 + 8713.5) + 10240.0 + 6248.0) * 1024,
                "residual": (2 * 14146.4 + 6248.0) * 1024}
 
@@ -142,6 +144,83 @@ def cpu_baseline(w, threads: int):
     return {"value": round(dt_cycle * w["n_vcycles"] * 1e3, 1), "unit": "ms per forward step",
             "cores": O.num_threads(), "kind": "port",
             "sample": f"1 V-cycle of step 2 of {w['name']} timed ({dt_cycle:.1f} s), x{w['n_vcycles']} V-cycles per step"}
+
+
+def unet_macs_per_pixel() -> float:
+    """Multiply-accumulates per finest-grid pixel of one inference of the
+    warm-start network, counted from its layer table: every 3x3 convolution
+    (9 x Cin x Cout at its level's resolution, 4^-level of the pixels) and
+    the 1x1 head."""
+    from glide_amd.warmstart import CH, HEAD_IN, LAYERS, LEVEL, N_OUT
+
+    total = 0.0
+    for name, ins, co in LAYERS:
+        total += 9 * sum(CH[i] for i in ins) * co / 4 ** LEVEL[name]
+    return total + CH[HEAD_IN] * N_OUT
+
+
+def unet_roofline(net, n=512, reps=10):
+    """One inference of the warm-start network on an n x n input, timed with
+    HIP events on the stream it runs on, priced at 2 x MACs against the
+    MI355X dense BF16 MFMA peak."""
+    import torch
+
+    from glide_amd.warmstart import IN_CH
+
+    st = torch.cuda.current_stream()
+    x = torch.zeros(n * n * IN_CH, dtype=torch.int16, device="cuda")
+    net.forward(x, 1, n, n)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        net.forward(x, 1, n, n)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    macs = unet_macs_per_pixel()
+    flops = 2.0 * macs * n * n
+    tf = flops / (ms * 1e-3) / 1e12
+    return {"kernel": f"warm-start U-Net inference, {n}^2 (13 BF16-MFMA 3x3 convolutions, pools, upsamplings, head)",
+            "bound": "mfma", "achieved": round(tf, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / PEAK_BF16_TFLOPS, 5), "ms_per_launch": round(ms, 4),
+            "work": f"{flops / 1e9:.1f} GFLOP = 2 x {macs:.0f} MACs per pixel x {n * n} pixels",
+            "note": "about 30 launches of 32- to 192-channel layers; launch- and activation-traffic-bound"}
+
+
+def cpu_report(threads_all: int):
+    """The CPU oracle (the reference has no CPU path) on configs 0 and 1:
+    ms per V-cycle and per 3-V-cycle step on one thread and on all the given
+    threads, with the host's CPU model."""
+    from glide_amd import synthetic as S
+    from oracle import oracle as O
+
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    rows = []
+    for name in ("dome_128", "cap_512"):
+        w = S.workload(name)
+        for threads in (1, threads_all):
+            O.set_threads(threads)
+            h = O.Hierarchy(w["ny"], w["nx"], w["dx"], n_levels=w["n_levels"])
+            h.set_geometry(w["bed"], w["H"])
+            h.set_parameters(B=w["B"], beta=w["beta"], smb=w["smb"])
+            h.forward(w["dt"], n_vcycles=1)
+            print(f"{name}: warm-up V-cycle on {threads} thread(s) done", file=sys.stderr, flush=True)
+            t0 = time.perf_counter()
+            h.forward(w["dt"], n_vcycles=1)
+            ms = (time.perf_counter() - t0) * 1e3
+            rows.append({"workload": name, "threads": O.num_threads(), "ms_per_vcycle": round(ms, 1),
+                         "ms_per_step": round(ms * w["n_vcycles"], 1)})
+            print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "rows": rows}
 
 
 def ensemble_bench(dt=10.0, n_vcycles=3):
@@ -382,6 +461,9 @@ def main():
     if args.reference_only:
         print(json.dumps(reference_schedule_bench()))
         return
+    if args.cpu_report:
+        print(json.dumps(cpu_report(args.cpu_threads or 16)))
+        return
     if args.kernels_only:
         from glide_amd import synthetic as S
         from glide_amd.physics import IcePhysics
@@ -426,6 +508,8 @@ def main():
         "all_converged": bool(all(r["plain_converged"] and r.get("learned_converged", True) for r in rows)),
         **rl,
     }
+    if learned:
+        line["unet_roofline"] = unet_roofline(net)
     line["learned_batched"] = learned_batched_bench(net, [r["seed"] for r in rows])
     line["reference_schedule"] = reference_schedule_bench()
     if not args.no_ensemble:

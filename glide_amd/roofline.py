@@ -18,6 +18,7 @@ NEWTON_ITERS = 10
 
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector and matrix)
+PEAK_BF16_TFLOPS = 2500.0    # MI355X dense BF16 MFMA
 
 
 def sweep_flops(n_cells: int, n_newton: int = NEWTON_ITERS) -> int:
