@@ -5,15 +5,18 @@
  * sigmoid (one exp) and Glen's-law viscosity (one pow).  Library versions of
  * these differ by an ulp between glibc on the host and ROCm's device library,
  * which would make host/device trajectories of the nonlinear solve drift
- * apart.  These routines use only IEEE-754 +, -, *, / (compiled without FMA
- * contraction), floor and integer bit operations, so they return the same
- * bits on an x86-64 host and on gfx950.
+ * apart.  These routines use only IEEE-754 +, -, *, / and fused multiply-add
+ * (each one correctly rounded; the code is compiled without FMA contraction,
+ * so only the explicit fma() calls fuse), floor and integer bit operations,
+ * so they return the same bits on an x86-64 host and on gfx950.  Each Horner
+ * step is one fma: half the dependent operations of a multiply and an add.
  *
  * Methods (textbook, independently derived):
- *   exp : Cody-Waite argument reduction x = k*ln2 + r, |r| <= ln2/2, with ln2
- *         split into a 21-bit head and a tail; degree-13 Taylor polynomial of
- *         e^r in Horner form (truncation < 5e-18 relative); exact scaling by
- *         2^k through the exponent field.
+ *   exp : Cody-Waite argument reduction x = k*ln2 + r, |r| <= ln2/2 (up to
+ *         an ulp of rounding in k), with ln2 split into a 21-bit head and a
+ *         tail; degree-13 Taylor polynomial of e^r in Horner form
+ *         (truncation < 5e-18 relative); exact scaling by 2^k through the
+ *         exponent field.
  *   log : x = m * 2^e with m in [sqrt(2)/2, sqrt(2)); f = m - 1 (exact),
  *         s = f / (2 + f), log(1 + f) = 2 atanh(s) written as
  *         f - (f^2/2 - s (f^2/2 + Q(s^2))) with Q the atanh series to s^22.
@@ -23,6 +26,7 @@
 #ifndef GLIDE_DETMATH_H
 #define GLIDE_DETMATH_H
 
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -53,23 +57,23 @@ DM_FN double dm_exp(double x)
     if (x != x) return x;
     if (x > 709.782712893384) return dm_from_bits(0x7ff0000000000000ULL);
     if (x < -745.1332191019412) return 0.0;
-    double kd = DM_FLOOR(x * INV_LN2 + 0.5);
+    double kd = DM_FLOOR(fma(x, INV_LN2, 0.5));
     int k = (int)kd;
-    double r = (x - kd * LN2_HI) - kd * LN2_LO;
+    double r = fma(-kd, LN2_LO, fma(-kd, LN2_HI, x));
     double p = 1.6059043836821613e-10;          /* 1/13! */
-    p = 2.08767569878681e-09 + r * p;           /* 1/12! */
-    p = 2.505210838544172e-08 + r * p;          /* 1/11! */
-    p = 2.755731922398589e-07 + r * p;          /* 1/10! */
-    p = 2.7557319223985893e-06 + r * p;         /* 1/9!  */
-    p = 2.48015873015873e-05 + r * p;           /* 1/8!  */
-    p = 0.0001984126984126984 + r * p;          /* 1/7!  */
-    p = 0.001388888888888889 + r * p;           /* 1/6!  */
-    p = 0.008333333333333333 + r * p;           /* 1/5!  */
-    p = 0.041666666666666664 + r * p;           /* 1/4!  */
-    p = 0.16666666666666666 + r * p;            /* 1/3!  */
-    p = 0.5 + r * p;                            /* 1/2!  */
-    p = 1.0 + r * p;                            /* 1/1!  */
-    p = 1.0 + r * p;
+    p = fma(r, p, 2.08767569878681e-09);           /* 1/12! */
+    p = fma(r, p, 2.505210838544172e-08);          /* 1/11! */
+    p = fma(r, p, 2.755731922398589e-07);          /* 1/10! */
+    p = fma(r, p, 2.7557319223985893e-06);         /* 1/9!  */
+    p = fma(r, p, 2.48015873015873e-05);           /* 1/8!  */
+    p = fma(r, p, 0.0001984126984126984);          /* 1/7!  */
+    p = fma(r, p, 0.001388888888888889);           /* 1/6!  */
+    p = fma(r, p, 0.008333333333333333);           /* 1/5!  */
+    p = fma(r, p, 0.041666666666666664);           /* 1/4!  */
+    p = fma(r, p, 0.16666666666666666);            /* 1/3!  */
+    p = fma(r, p, 0.5);                            /* 1/2!  */
+    p = fma(r, p, 1.0);                            /* 1/1!  */
+    p = fma(r, p, 1.0);
     if (k > 1023) return (p * dm_pow2i(1023)) * dm_pow2i(k - 1023);
     if (k < -1022) return (p * dm_pow2i(k + 1000)) * dm_pow2i(-1000);
     return p * dm_pow2i(k);
@@ -98,21 +102,21 @@ DM_FN double dm_log(double x)
     double s = f / (2.0 + f);
     double z = s * s;
     double Q = 0.08695652173913043;             /* 2/23 */
-    Q = 0.09523809523809523 + z * Q;            /* 2/21 */
-    Q = 0.10526315789473684 + z * Q;            /* 2/19 */
-    Q = 0.11764705882352941 + z * Q;            /* 2/17 */
-    Q = 0.13333333333333333 + z * Q;            /* 2/15 */
-    Q = 0.15384615384615385 + z * Q;            /* 2/13 */
-    Q = 0.18181818181818182 + z * Q;            /* 2/11 */
-    Q = 0.2222222222222222 + z * Q;             /* 2/9  */
-    Q = 0.2857142857142857 + z * Q;             /* 2/7  */
-    Q = 0.4 + z * Q;                            /* 2/5  */
-    Q = 0.6666666666666666 + z * Q;             /* 2/3  */
+    Q = fma(z, Q, 0.09523809523809523);            /* 2/21 */
+    Q = fma(z, Q, 0.10526315789473684);            /* 2/19 */
+    Q = fma(z, Q, 0.11764705882352941);            /* 2/17 */
+    Q = fma(z, Q, 0.13333333333333333);            /* 2/15 */
+    Q = fma(z, Q, 0.15384615384615385);            /* 2/13 */
+    Q = fma(z, Q, 0.18181818181818182);            /* 2/11 */
+    Q = fma(z, Q, 0.2222222222222222);             /* 2/9  */
+    Q = fma(z, Q, 0.2857142857142857);             /* 2/7  */
+    Q = fma(z, Q, 0.4);                            /* 2/5  */
+    Q = fma(z, Q, 0.6666666666666666);             /* 2/3  */
     Q = z * Q;
     double hfsq = 0.5 * f * f;
-    double lm = f - (hfsq - s * (hfsq + Q));
+    double lm = f - fma(-s, hfsq + Q, hfsq);
     double ed = (double)e;
-    return ed * LN2_HI + (ed * LN2_LO + lm);
+    return fma(ed, LN2_HI, fma(ed, LN2_LO, lm));
 }
 
 /* x^p for x >= 0 (x > 0 on every call site of the solver) */

@@ -6,8 +6,10 @@
 // (glide/cuda/stress.cu, flux.cu, viscosity.cu, common.cu).  The kernels
 // compose them; out-of-domain terms return all-zero rows exactly where the
 // reference's do.  Compiled with -ffp-contract=off, so every product and sum
-// rounds as written (the dual-number product uses an explicit fma, as the
-// reference's does, common.cu:20).
+// rounds as written except where an explicit fma() fuses them: at the
+// multiply-adds that the reference's --use_fast_math build contracts
+// (kernels.py:72), written out so that the host oracle rounds identically
+// (the dual-number product's fma is explicit in the reference, common.cu:20).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -78,15 +80,15 @@ __host__ __device__ __forceinline__ double eta_cell_acc(int i, int j, const UA &
     double dudx = (u_r - u_l) * dx_inv;
     double dvdy = (v_t - v_b) * dx_inv;
     double tl_mask = (i > 0 && j > 0);
-    double e_tl = 0.5 * ((U(i - 1, j) - u_l) * dx_inv + (v_t - V(i, j - 1)) * dx_inv) * tl_mask;
+    double e_tl = 0.5 * fma(U(i - 1, j) - u_l, dx_inv, (v_t - V(i, j - 1)) * dx_inv) * tl_mask;
     double tr_mask = (i > 0 && j < (nx - 1));
-    double e_tr = 0.5 * ((U(i - 1, j + 1) - u_r) * dx_inv + (V(i, j + 1) - v_t) * dx_inv) * tr_mask;
+    double e_tr = 0.5 * fma(U(i - 1, j + 1) - u_r, dx_inv, (V(i, j + 1) - v_t) * dx_inv) * tr_mask;
     double bl_mask = (i < (ny - 1) && j > 0);
-    double e_bl = 0.5 * ((u_l - U(i + 1, j)) * dx_inv + (v_b - V(i + 1, j - 1)) * dx_inv) * bl_mask;
+    double e_bl = 0.5 * fma(u_l - U(i + 1, j), dx_inv, (v_b - V(i + 1, j - 1)) * dx_inv) * bl_mask;
     double br_mask = (i < (ny - 1) && j < (nx - 1));
-    double e_br = 0.5 * ((u_r - U(i + 1, j + 1)) * dx_inv + (V(i + 1, j + 1) - v_b) * dx_inv) * br_mask;
-    double exy2 = 0.25 * (e_tl * e_tl + e_tr * e_tr + e_bl * e_bl + e_br * e_br);
-    double eII = dudx * dudx + dvdy * dvdy + dudx * dvdy + exy2 + p.eps_reg;
+    double e_br = 0.5 * fma(u_r - U(i + 1, j + 1), dx_inv, (V(i + 1, j + 1) - v_b) * dx_inv) * br_mask;
+    double exy2 = 0.25 * fma(e_br, e_br, fma(e_bl, e_bl, fma(e_tr, e_tr, e_tl * e_tl)));
+    double eII = fma(dudx, dvdy, fma(dvdy, dvdy, dudx * dudx)) + exy2 + p.eps_reg;
     return 0.5 * Bc * dm_pow(eII, glen_exp);
 }
 __host__ __device__ __forceinline__ double eta_cell(int i, int j, const double *__restrict__ u,
@@ -149,7 +151,7 @@ __host__ __device__ __forceinline__ SigN sigma_xx(double u_l, double u_r, double
 {
     SigN J = {0, 0, 0, 0, 0, 0};
     if (j < 0 || j >= nx) return J;
-    double eps_xx = (2.0 * (u_r - u_l) * dx_inv + (v_t - v_b) * dx_inv);
+    double eps_xx = fma(2.0 * (u_r - u_l), dx_inv, (v_t - v_b) * dx_inv);
     double pre = 2.0 * eta_H * dx_inv;
     J.res = 2.0 * eta_H * eps_xx;
     J.d_u_l = -2.0 * pre;
@@ -165,7 +167,7 @@ __host__ __device__ __forceinline__ SigN sigma_yy(double u_l, double u_r, double
 {
     SigN J = {0, 0, 0, 0, 0, 0};
     if (i < 0 || i >= ny) return J;
-    double eps_yy = ((u_r - u_l) * dx_inv + 2.0 * (v_t - v_b) * dx_inv);
+    double eps_yy = fma(u_r - u_l, dx_inv, 2.0 * (v_t - v_b) * dx_inv);
     double pre = 2.0 * eta_H * dx_inv;
     J.res = 2.0 * eta_H * eps_yy;
     J.d_u_l = -pre;
@@ -181,7 +183,7 @@ __host__ __device__ __forceinline__ SigS sigma_xy(double u_t, double u_b, double
 {
     SigS J = {0, 0, 0, 0, 0, 0};
     if (i <= 0 || i >= ny || j <= 0 || j >= nx) return J;
-    double eps_xy = 0.5 * ((u_t - u_b) * dx_inv + (v_r - v_l) * dx_inv);
+    double eps_xy = 0.5 * fma(u_t - u_b, dx_inv, (v_r - v_l) * dx_inv);
     double pre = eta_H * dx_inv;
     J.res = 2.0 * eta_H * eps_xy;
     J.d_u_t = pre;
@@ -198,8 +200,8 @@ __host__ __device__ __forceinline__ TauB tau_b_g(double vel, double g_l, double 
                                                  double beta_l, double beta_r, const Params &p)
 {
     TauB J = {0, 0, 0, 0, 0, 0};
-    double be_l = g_l * beta_l + (1.0 - g_l) * p.water_drag;
-    double be_r = g_r * beta_r + (1.0 - g_r) * p.water_drag;
+    double be_l = fma(g_l, beta_l, (1.0 - g_l) * p.water_drag);
+    double be_r = fma(g_r, beta_r, (1.0 - g_r) * p.water_drag);
     double be = 0.5 * (be_l + be_r);
     J.res = -be * vel;
     J.d_u = -be;
@@ -219,7 +221,7 @@ __host__ __device__ __forceinline__ TauB tau_b_g(double vel, double g_l, double 
 struct Ground { double g, gd; };
 __host__ __device__ __forceinline__ Ground ground(double bed, double H, const Params &p)
 {
-    double z = bed + 0.917 * H;
+    double z = fma(0.917, H, bed);
     Ground o;
     o.g = sigm(z, p.gl_c);
     o.gd = p.gl_derivatives ? sigm_d(z, p.gl_c) : 0.0;
@@ -245,8 +247,8 @@ __host__ __device__ __forceinline__ TauD tau_dx(double H_l, double H_r, double b
     double S_l = base_l + H_l;
     double S_r = base_r + H_r;
     J.res = H_avg * (S_r - S_l) * dx_inv;
-    J.d_H_l = 0.5 * (S_r - S_l) * dx_inv - H_avg * (1.0 + db_l) * dx_inv;
-    J.d_H_r = 0.5 * (S_r - S_l) * dx_inv + H_avg * (1.0 + db_r) * dx_inv;
+    J.d_H_l = fma(-(H_avg * (1.0 + db_l)), dx_inv, 0.5 * (S_r - S_l) * dx_inv);
+    J.d_H_r = fma(H_avg * (1.0 + db_r), dx_inv, 0.5 * (S_r - S_l) * dx_inv);
     return J;
 }
 // Driving stress on v face row i, cells t (above) and b (below); d_H_l = d/dH_t
@@ -264,8 +266,8 @@ __host__ __device__ __forceinline__ TauD tau_dy(double H_t, double H_b, double b
     double S_t = base_t + H_t;
     double S_b = base_b + H_b;
     J.res = H_avg * (S_t - S_b) * dx_inv;
-    J.d_H_l = 0.5 * (S_t - S_b) * dx_inv + H_avg * (1.0 + db_t) * dx_inv;
-    J.d_H_r = 0.5 * (S_t - S_b) * dx_inv - H_avg * (1.0 + db_b) * dx_inv;
+    J.d_H_l = fma(H_avg * (1.0 + db_t), dx_inv, 0.5 * (S_t - S_b) * dx_inv);
+    J.d_H_r = fma(-(H_avg * (1.0 + db_b)), dx_inv, 0.5 * (S_t - S_b) * dx_inv);
     return J;
 }
 // Upwind mass flux through u face column j (flux.cu:40-64)
@@ -276,10 +278,10 @@ __host__ __device__ __forceinline__ Flux flux_x(double u, double H_l, double H_r
     double H_avg = 0.5 * (H_l + H_r);
     double um = fabs(u);
     double us = copysign(1.0, u);
-    J.res = H_avg * u - 0.5 * um * (H_r - H_l);
+    J.res = fma(H_avg, u, -(0.5 * um * (H_r - H_l)));
     J.d_H_l = 0.5 * (u + um);
     J.d_H_r = 0.5 * (u - um);
-    J.d_u = H_avg - 0.5 * us * (H_r - H_l);
+    J.d_u = fma(-(0.5 * us), H_r - H_l, H_avg);
     return J;
 }
 // Upwind mass flux through v face row i; d_H_l = d/dH_t, d_H_r = d/dH_b (flux.cu:111-135)
@@ -290,10 +292,10 @@ __host__ __device__ __forceinline__ Flux flux_y(double v, double H_t, double H_b
     double H_avg = 0.5 * (H_t + H_b);
     double vm = fabs(v);
     double vs = copysign(1.0, v);
-    J.res = H_avg * v - 0.5 * vm * (H_t - H_b);
+    J.res = fma(H_avg, v, -(0.5 * vm * (H_t - H_b)));
     J.d_H_l = 0.5 * (v - vm);
     J.d_H_r = 0.5 * (v + vm);
-    J.d_u = H_avg - 0.5 * vs * (H_t - H_b);
+    J.d_u = fma(-(0.5 * vs), H_t - H_b, H_avg);
     return J;
 }
 // Calving sink of floating ice (flux.cu:188-211), given the cell's sigmoid
@@ -304,7 +306,7 @@ __host__ __device__ __forceinline__ Calv calving_g(double H, const Ground &gr, c
     J.d_H = -p.calving_rate * (1.0 - gr.g);
     if (p.gl_derivatives) {
         double dg = 0.917 * gr.gd;
-        J.d_H += p.calving_rate * H * dg;
+        J.d_H = fma(p.calving_rate * H, dg, J.d_H);
     }
     return J;
 }

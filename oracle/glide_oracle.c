@@ -9,8 +9,12 @@
  * Every function follows the reference's algorithm in FP64 with the
  * reference's operation order (the reference computes in FP32 with
  * --use_fast_math, glide/kernels.py:72).  Built with -ffp-contract=off so no
- * multiply-add is fused behind the source's back; the dual-number product
- * uses an explicit fma() because the reference's does (common.cu:20).
+ * multiply-add is fused behind the source's back.  Where the fast-math build
+ * contracts a product and a sum (the LU updates, the Horner steps of exp and
+ * log, the Vanka block's accumulations, the stress, drag, flux and viscosity
+ * sums), this restatement writes an explicit fma(), at the same places as the
+ * device code, so both round identically; the dual-number product's fma() is
+ * explicit in the reference too (common.cu:20).
  *
  * Transcendentals: by default the deterministic exp/log/pow of
  * glide_amd/csrc/detmath.h (same bits on host and gfx950); building with
@@ -117,18 +121,18 @@ static double eta_cell(int i, int j, const double *u, const double *v, const dou
     double dvdy = (v_t - v_b) * dx_inv;
     double tl_mask = (i > 0 && j > 0);
     double u_tl = VF(u, i - 1, j, ny, nx), v_lt = HF(v, i, j - 1, ny, nx);
-    double e_tl = 0.5 * ((u_tl - u_l) * dx_inv + (v_t - v_lt) * dx_inv) * tl_mask;
+    double e_tl = 0.5 * fma(u_tl - u_l, dx_inv, (v_t - v_lt) * dx_inv) * tl_mask;
     double tr_mask = (i > 0 && j < (nx - 1));
     double u_tr = VF(u, i - 1, j + 1, ny, nx), v_rt = HF(v, i, j + 1, ny, nx);
-    double e_tr = 0.5 * ((u_tr - u_r) * dx_inv + (v_rt - v_t) * dx_inv) * tr_mask;
+    double e_tr = 0.5 * fma(u_tr - u_r, dx_inv, (v_rt - v_t) * dx_inv) * tr_mask;
     double bl_mask = (i < (ny - 1) && j > 0);
     double u_bl = VF(u, i + 1, j, ny, nx), v_lb = HF(v, i + 1, j - 1, ny, nx);
-    double e_bl = 0.5 * ((u_l - u_bl) * dx_inv + (v_b - v_lb) * dx_inv) * bl_mask;
+    double e_bl = 0.5 * fma(u_l - u_bl, dx_inv, (v_b - v_lb) * dx_inv) * bl_mask;
     double br_mask = (i < (ny - 1) && j < (nx - 1));
     double u_br = VF(u, i + 1, j + 1, ny, nx), v_rb = HF(v, i + 1, j + 1, ny, nx);
-    double e_br = 0.5 * ((u_r - u_br) * dx_inv + (v_rb - v_b) * dx_inv) * br_mask;
-    double exy2 = 0.25 * (e_tl * e_tl + e_tr * e_tr + e_bl * e_bl + e_br * e_br);
-    double eII = dudx * dudx + dvdy * dvdy + dudx * dvdy + exy2 + eps_reg;
+    double e_br = 0.5 * fma(u_r - u_br, dx_inv, (v_rb - v_b) * dx_inv) * br_mask;
+    double exy2 = 0.25 * fma(e_br, e_br, fma(e_bl, e_bl, fma(e_tr, e_tr, e_tl * e_tl)));
+    double eII = fma(dudx, dvdy, fma(dvdy, dvdy, dudx * dudx)) + exy2 + eps_reg;
     return 0.5 * CE(B, i, j, ny, nx) * OPOW(eII, glen_exp);
 }
 
@@ -187,7 +191,7 @@ static inline SigN sigma_xx(double u_l, double u_r, double v_t, double v_b, doub
     SigN J = {0, 0, 0, 0, 0, 0};
     (void)i; (void)ny;
     if (j < 0 || j >= nx) return J;
-    double eps_xx = (2.0 * (u_r - u_l) * dx_inv + (v_t - v_b) * dx_inv);
+    double eps_xx = fma(2.0 * (u_r - u_l), dx_inv, (v_t - v_b) * dx_inv);
     double pre = 2.0 * eta_H * dx_inv;
     J.res = 2.0 * eta_H * eps_xx;
     J.d_u_l = -2.0 * pre;
@@ -204,7 +208,7 @@ static inline SigN sigma_yy(double u_l, double u_r, double v_t, double v_b, doub
     SigN J = {0, 0, 0, 0, 0, 0};
     (void)j; (void)nx;
     if (i < 0 || i >= ny) return J;
-    double eps_yy = ((u_r - u_l) * dx_inv + 2.0 * (v_t - v_b) * dx_inv);
+    double eps_yy = fma(u_r - u_l, dx_inv, 2.0 * (v_t - v_b) * dx_inv);
     double pre = 2.0 * eta_H * dx_inv;
     J.res = 2.0 * eta_H * eps_yy;
     J.d_u_l = -pre;
@@ -220,7 +224,7 @@ static inline SigS sigma_xy(double u_t, double u_b, double v_l, double v_r, doub
 {
     SigS J = {0, 0, 0, 0, 0, 0};
     if (i <= 0 || i >= ny || j <= 0 || j >= nx) return J;
-    double eps_xy = 0.5 * ((u_t - u_b) * dx_inv + (v_r - v_l) * dx_inv);
+    double eps_xy = 0.5 * fma(u_t - u_b, dx_inv, (v_r - v_l) * dx_inv);
     double pre = eta_H * dx_inv;
     J.res = 2.0 * eta_H * eps_xy;
     J.d_u_t = pre;
@@ -236,12 +240,12 @@ static inline TauB tau_b(double vel, double H_l, double H_r, double bed_l, doubl
                          double beta_l, double beta_r, const OParams *p)
 {
     TauB J = {0, 0, 0, 0, 0, 0};
-    double z_l = bed_l + 0.917 * H_l;
-    double z_r = bed_r + 0.917 * H_r;
+    double z_l = fma(0.917, H_l, bed_l);
+    double z_r = fma(0.917, H_r, bed_r);
     double g_l = sigm(z_l, p->gl_sigmoid_c);
     double g_r = sigm(z_r, p->gl_sigmoid_c);
-    double be_l = g_l * beta_l + (1.0 - g_l) * p->water_drag;
-    double be_r = g_r * beta_r + (1.0 - g_r) * p->water_drag;
+    double be_l = fma(g_l, beta_l, (1.0 - g_l) * p->water_drag);
+    double be_r = fma(g_r, beta_r, (1.0 - g_r) * p->water_drag);
     double be = 0.5 * (be_l + be_r);
     J.res = -be * vel;
     J.d_u = -be;
@@ -272,8 +276,8 @@ static inline TauD tau_dx(double H_l, double H_r, double bed_l, double bed_r,
     double S_l = base_l + H_l;
     double S_r = base_r + H_r;
     J.res = H_avg * (S_r - S_l) * dx_inv;
-    J.d_H_l = 0.5 * (S_r - S_l) * dx_inv - H_avg * (1.0 + db_l) * dx_inv;
-    J.d_H_r = 0.5 * (S_r - S_l) * dx_inv + H_avg * (1.0 + db_r) * dx_inv;
+    J.d_H_l = fma(-(H_avg * (1.0 + db_l)), dx_inv, 0.5 * (S_r - S_l) * dx_inv);
+    J.d_H_r = fma(H_avg * (1.0 + db_r), dx_inv, 0.5 * (S_r - S_l) * dx_inv);
     return J;
 }
 /* stress.cu:482-510: face (i, j) of the v grid, t/b cells; d_H_l=d_H_t, d_H_r=d_H_b */
@@ -291,8 +295,8 @@ static inline TauD tau_dy(double H_t, double H_b, double bed_t, double bed_b,
     double S_t = base_t + H_t;
     double S_b = base_b + H_b;
     J.res = H_avg * (S_t - S_b) * dx_inv;
-    J.d_H_l = 0.5 * (S_t - S_b) * dx_inv + H_avg * (1.0 + db_t) * dx_inv;
-    J.d_H_r = 0.5 * (S_t - S_b) * dx_inv - H_avg * (1.0 + db_b) * dx_inv;
+    J.d_H_l = fma(H_avg * (1.0 + db_t), dx_inv, 0.5 * (S_t - S_b) * dx_inv);
+    J.d_H_r = fma(-(H_avg * (1.0 + db_b)), dx_inv, 0.5 * (S_t - S_b) * dx_inv);
     return J;
 }
 
@@ -308,10 +312,10 @@ static inline Flux flux_x(double u, double H_l, double H_r, int i, int j, int ny
     double H_avg = 0.5 * (H_l + H_r);
     double um = fabs(u);
     double us = copysign(1.0, u);
-    J.res = H_avg * u - 0.5 * um * (H_r - H_l);
+    J.res = fma(H_avg, u, -(0.5 * um * (H_r - H_l)));
     J.d_H_l = 0.5 * (u + um);
     J.d_H_r = 0.5 * (u - um);
-    J.d_u = H_avg - 0.5 * us * (H_r - H_l);
+    J.d_u = fma(-(0.5 * us), H_r - H_l, H_avg);
     return J;
 }
 /* flux.cu:111-135: v face (i, j), cells t = (i-1, j), b = (i, j); d_H_l=d_H_t, d_H_r=d_H_b */
@@ -323,23 +327,23 @@ static inline Flux flux_y(double v, double H_t, double H_b, int i, int j, int ny
     double H_avg = 0.5 * (H_t + H_b);
     double vm = fabs(v);
     double vs = copysign(1.0, v);
-    J.res = H_avg * v - 0.5 * vm * (H_t - H_b);
+    J.res = fma(H_avg, v, -(0.5 * vm * (H_t - H_b)));
     J.d_H_l = 0.5 * (v - vm);
     J.d_H_r = 0.5 * (v + vm);
-    J.d_u = H_avg - 0.5 * vs * (H_t - H_b);
+    J.d_u = fma(-(0.5 * vs), H_t - H_b, H_avg);
     return J;
 }
 /* flux.cu:188-211 */
 static inline Calv calving(double H, double bed, const OParams *p)
 {
     Calv J;
-    double z = bed + 0.917 * H;
+    double z = fma(0.917, H, bed);
     double g = sigm(z, p->gl_sigmoid_c);
     J.res = -p->calving_rate * (1.0 - g) * H;
     J.d_H = -p->calving_rate * (1.0 - g);
     if (p->gl_derivatives) {
         double dg = 0.917 * sigm_d(z, p->gl_sigmoid_c);
-        J.d_H += p->calving_rate * H * dg;
+        J.d_H = fma(p->calving_rate * H, dg, J.d_H);
     }
     return J;
 }
@@ -891,31 +895,36 @@ void or_vjp(const OGeom *g, const OParams *p, double *l, const double *U, const 
 }
 
 /* ------------------------------------------------------------------ */
-/* 5x5 Doolittle LU without pivoting (common.cu:132-175)               */
+/* 5x5 Doolittle LU without pivoting (common.cu:132-175).  The          */
+/* reference's module is built with --use_fast_math (kernels.py:72):    */
+/* contracted multiply-adds, reciprocal-based division.  Here every     */
+/* update c - a*b is one fma and back substitution multiplies by the    */
+/* elimination's pivot reciprocals; substitution sums the terms of the  */
+/* later unknowns first.                                                */
 /* ------------------------------------------------------------------ */
 void or_lu5(const double *A, const double *b, double *x)
 {
-    double LU[5][5];
+    double LU[5][5], inv[5];
     for (int i = 0; i < 5; i++)
         for (int j = 0; j < 5; j++) LU[i][j] = A[i * 5 + j];
     for (int k = 0; k < 5; k++) {
-        double inv = 1.0 / LU[k][k];
+        inv[k] = 1.0 / LU[k][k];
         for (int i = k + 1; i < 5; i++) {
-            LU[i][k] *= inv;
-            for (int j = k + 1; j < 5; j++) LU[i][j] -= LU[i][k] * LU[k][j];
+            LU[i][k] *= inv[k];
+            for (int j = k + 1; j < 5; j++) LU[i][j] = fma(-LU[i][k], LU[k][j], LU[i][j]);
         }
     }
     double y[5];
     y[0] = b[0];
-    y[1] = b[1] - LU[1][0] * y[0];
-    y[2] = b[2] - LU[2][0] * y[0] - LU[2][1] * y[1];
-    y[3] = b[3] - LU[3][0] * y[0] - LU[3][1] * y[1] - LU[3][2] * y[2];
-    y[4] = b[4] - LU[4][0] * y[0] - LU[4][1] * y[1] - LU[4][2] * y[2] - LU[4][3] * y[3];
-    x[4] = y[4] / LU[4][4];
-    x[3] = (y[3] - LU[3][4] * x[4]) / LU[3][3];
-    x[2] = (y[2] - LU[2][3] * x[3] - LU[2][4] * x[4]) / LU[2][2];
-    x[1] = (y[1] - LU[1][2] * x[2] - LU[1][3] * x[3] - LU[1][4] * x[4]) / LU[1][1];
-    x[0] = (y[0] - LU[0][1] * x[1] - LU[0][2] * x[2] - LU[0][3] * x[3] - LU[0][4] * x[4]) / LU[0][0];
+    y[1] = fma(-LU[1][0], y[0], b[1]);
+    y[2] = fma(-LU[2][1], y[1], fma(-LU[2][0], y[0], b[2]));
+    y[3] = fma(-LU[3][2], y[2], fma(-LU[3][1], y[1], fma(-LU[3][0], y[0], b[3])));
+    y[4] = fma(-LU[4][3], y[3], fma(-LU[4][2], y[2], fma(-LU[4][1], y[1], fma(-LU[4][0], y[0], b[4]))));
+    x[4] = y[4] * inv[4];
+    x[3] = fma(-LU[3][4], x[4], y[3]) * inv[3];
+    x[2] = fma(-LU[2][3], x[3], fma(-LU[2][4], x[4], y[2])) * inv[2];
+    x[1] = fma(-LU[1][2], x[2], fma(-LU[1][3], x[3], fma(-LU[1][4], x[4], y[1]))) * inv[1];
+    x[0] = fma(-LU[0][1], x[1], fma(-LU[0][2], x[2], fma(-LU[0][3], x[3], fma(-LU[0][4], x[4], y[0])))) * inv[0];
 }
 
 /* ------------------------------------------------------------------ */
@@ -963,24 +972,24 @@ static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, 
         r[4] -= jc.res;
         double H_l = CE(H, i, j - 1, ny, nx);
         Flux q = flux_x(u_l, H_l, H_c, i, j, ny, nx);
-        J[20] -= q.d_u * dx_inv;
-        J[24] -= q.d_H_r * dx_inv;
-        r[4] -= q.res * dx_inv;
+        J[20] = fma(-q.d_u, dx_inv, J[20]);
+        J[24] = fma(-q.d_H_r, dx_inv, J[24]);
+        r[4] = fma(-q.res, dx_inv, r[4]);
         double H_r = CE(H, i, j + 1, ny, nx);
         q = flux_x(u_r, H_c, H_r, i, j + 1, ny, nx);
-        J[21] += q.d_u * dx_inv;
-        J[24] += q.d_H_l * dx_inv;
-        r[4] += q.res * dx_inv;
+        J[21] = fma(q.d_u, dx_inv, J[21]);
+        J[24] = fma(q.d_H_l, dx_inv, J[24]);
+        r[4] = fma(q.res, dx_inv, r[4]);
         double H_t = CE(H, i - 1, j, ny, nx);
         q = flux_y(v_t, H_t, H_c, i, j, ny, nx);
-        J[22] += q.d_u * dx_inv;
-        J[24] += q.d_H_r * dx_inv;
-        r[4] += q.res * dx_inv;
+        J[22] = fma(q.d_u, dx_inv, J[22]);
+        J[24] = fma(q.d_H_r, dx_inv, J[24]);
+        r[4] = fma(q.res, dx_inv, r[4]);
         double H_b = CE(H, i + 1, j, ny, nx);
         q = flux_y(v_b, H_c, H_b, i + 1, j, ny, nx);
-        J[23] -= q.d_u * dx_inv;
-        J[24] -= q.d_H_l * dx_inv;
-        r[4] -= q.res * dx_inv;
+        J[23] = fma(-q.d_u, dx_inv, J[23]);
+        J[24] = fma(-q.d_H_l, dx_inv, J[24]);
+        r[4] = fma(-q.res, dx_inv, r[4]);
         int constrained = (mode == 0) ? ((H_c - dt * r[4]) <= thklim) : (adj_masked != 0);
         if (constrained) {
             for (int k = 0; k < 5; k++) J[20 + k] = 0.0;
@@ -993,51 +1002,51 @@ static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, 
         double ec = ETA(eta, i, j, ny, nx);
         double eH = H_c * ec;   /* get_eta_H_cell_jac: res = H*eta, d_H = eta */
         SigN s = sigma_xx(u_l, u_r, v_t, v_b, eH, dx_inv, i, j, ny, nx);
-        r[0] += s.res * dx_inv;
-        J[0] += s.d_u_l * dx_inv; J[1] += s.d_u_r * dx_inv; J[2] += s.d_v_t * dx_inv;
-        J[3] += s.d_v_b * dx_inv; J[4] += s.d_eta_H * ec * dx_inv;
-        r[1] -= s.res * dx_inv;
-        J[5] -= s.d_u_l * dx_inv; J[6] -= s.d_u_r * dx_inv; J[7] -= s.d_v_t * dx_inv;
-        J[8] -= s.d_v_b * dx_inv; J[9] -= s.d_eta_H * ec * dx_inv;
+        r[0] = fma(s.res, dx_inv, r[0]);
+        J[0] = fma(s.d_u_l, dx_inv, J[0]); J[1] = fma(s.d_u_r, dx_inv, J[1]); J[2] = fma(s.d_v_t, dx_inv, J[2]);
+        J[3] = fma(s.d_v_b, dx_inv, J[3]); J[4] = fma(s.d_eta_H * ec, dx_inv, J[4]);
+        r[1] = fma(-s.res, dx_inv, r[1]);
+        J[5] = fma(-s.d_u_l, dx_inv, J[5]); J[6] = fma(-s.d_u_r, dx_inv, J[6]); J[7] = fma(-s.d_v_t, dx_inv, J[7]);
+        J[8] = fma(-s.d_v_b, dx_inv, J[8]); J[9] = fma(-(s.d_eta_H * ec), dx_inv, J[9]);
         SigN t = sigma_yy(u_l, u_r, v_t, v_b, eH, dx_inv, i, j, ny, nx);
-        r[2] -= t.res * dx_inv;
-        J[10] -= t.d_u_l * dx_inv; J[11] -= t.d_u_r * dx_inv; J[12] -= t.d_v_t * dx_inv;
-        J[13] -= t.d_v_b * dx_inv; J[14] -= t.d_eta_H * ec * dx_inv;
-        r[3] += t.res * dx_inv;
-        J[15] += t.d_u_l * dx_inv; J[16] += t.d_u_r * dx_inv; J[17] += t.d_v_t * dx_inv;
-        J[18] += t.d_v_b * dx_inv; J[19] += t.d_eta_H * ec * dx_inv;
+        r[2] = fma(-t.res, dx_inv, r[2]);
+        J[10] = fma(-t.d_u_l, dx_inv, J[10]); J[11] = fma(-t.d_u_r, dx_inv, J[11]); J[12] = fma(-t.d_v_t, dx_inv, J[12]);
+        J[13] = fma(-t.d_v_b, dx_inv, J[13]); J[14] = fma(-(t.d_eta_H * ec), dx_inv, J[14]);
+        r[3] = fma(t.res, dx_inv, r[3]);
+        J[15] = fma(t.d_u_l, dx_inv, J[15]); J[16] = fma(t.d_u_r, dx_inv, J[16]); J[17] = fma(t.d_v_t, dx_inv, J[17]);
+        J[18] = fma(t.d_v_b, dx_inv, J[18]); J[19] = fma(t.d_eta_H * ec, dx_inv, J[19]);
     }
     /* left cell sigma_xx into the left u row */
     {
         double eH = ETA(eta, i, j - 1, ny, nx) * CE(H, i, j - 1, ny, nx);
         SigN s = sigma_xx(VF(u, i, j - 1, ny, nx), u_l, HF(v, i, j - 1, ny, nx), HF(v, i + 1, j - 1, ny, nx),
                           eH, dx_inv, i, j - 1, ny, nx);
-        r[0] -= s.res * dx_inv;
-        J[0] -= s.d_u_r * dx_inv;
+        r[0] = fma(-s.res, dx_inv, r[0]);
+        J[0] = fma(-s.d_u_r, dx_inv, J[0]);
     }
     /* right cell sigma_xx into the right u row */
     {
         double eH = ETA(eta, i, j + 1, ny, nx) * CE(H, i, j + 1, ny, nx);
         SigN s = sigma_xx(u_r, VF(u, i, j + 2, ny, nx), HF(v, i, j + 1, ny, nx), HF(v, i + 1, j + 1, ny, nx),
                           eH, dx_inv, i, j + 1, ny, nx);
-        r[1] += s.res * dx_inv;
-        J[6] += s.d_u_l * dx_inv;
+        r[1] = fma(s.res, dx_inv, r[1]);
+        J[6] = fma(s.d_u_l, dx_inv, J[6]);
     }
     /* top cell sigma_yy into the top v row */
     {
         double eH = ETA(eta, i - 1, j, ny, nx) * CE(H, i - 1, j, ny, nx);
         SigN s = sigma_yy(VF(u, i - 1, j, ny, nx), VF(u, i - 1, j + 1, ny, nx), HF(v, i - 1, j, ny, nx), v_t,
                           eH, dx_inv, i - 1, j, ny, nx);
-        r[2] += s.res * dx_inv;
-        J[12] += s.d_v_b * dx_inv;
+        r[2] = fma(s.res, dx_inv, r[2]);
+        J[12] = fma(s.d_v_b, dx_inv, J[12]);
     }
     /* bottom cell sigma_yy into the bottom v row */
     {
         double eH = ETA(eta, i + 1, j, ny, nx) * CE(H, i + 1, j, ny, nx);
         SigN s = sigma_yy(VF(u, i + 1, j, ny, nx), VF(u, i + 1, j + 1, ny, nx), v_b, HF(v, i + 2, j, ny, nx),
                           eH, dx_inv, i + 1, j, ny, nx);
-        r[3] -= s.res * dx_inv;
-        J[18] -= s.d_v_t * dx_inv;
+        r[3] = fma(-s.res, dx_inv, r[3]);
+        J[18] = fma(-s.d_v_t, dx_inv, J[18]);
     }
     /* top-left vertex (i, j) */
     {
@@ -1047,13 +1056,13 @@ static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, 
                                 CE(H, i, j - 1, ny, nx), H_c);
         double dH_br = 0.25 * e_c;
         SigS s = sigma_xy(VF(u, i - 1, j, ny, nx), u_l, HF(v, i, j - 1, ny, nx), v_t, eH, dx_inv, i, j, ny, nx);
-        r[0] += s.res * dx_inv;
-        J[0] += s.d_u_b * dx_inv;
-        J[4] += s.d_eta_H * dH_br * dx_inv;
-        r[2] -= s.res * dx_inv;
-        J[12] -= s.d_v_r * dx_inv;
-        J[14] -= s.d_eta_H * dH_br * dx_inv;
-        if (g_full_block) { J[2] += s.d_v_r * dx_inv; J[10] -= s.d_u_b * dx_inv; }
+        r[0] = fma(s.res, dx_inv, r[0]);
+        J[0] = fma(s.d_u_b, dx_inv, J[0]);
+        J[4] = fma(s.d_eta_H * dH_br, dx_inv, J[4]);
+        r[2] = fma(-s.res, dx_inv, r[2]);
+        J[12] = fma(-s.d_v_r, dx_inv, J[12]);
+        J[14] = fma(-(s.d_eta_H * dH_br), dx_inv, J[14]);
+        if (g_full_block) { J[2] = fma(s.d_v_r, dx_inv, J[2]); J[10] = fma(-s.d_u_b, dx_inv, J[10]); }
     }
     /* top-right vertex (i, j+1) */
     {
@@ -1063,13 +1072,13 @@ static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, 
                                 H_c, CE(H, i, j + 1, ny, nx));
         double dH_bl = 0.25 * e_c;
         SigS s = sigma_xy(VF(u, i - 1, j + 1, ny, nx), u_r, v_t, HF(v, i, j + 1, ny, nx), eH, dx_inv, i, j + 1, ny, nx);
-        r[1] += s.res * dx_inv;
-        J[6] += s.d_u_b * dx_inv;
-        J[9] += s.d_eta_H * dH_bl * dx_inv;
-        r[2] += s.res * dx_inv;
-        J[12] += s.d_v_l * dx_inv;
-        J[14] += s.d_eta_H * dH_bl * dx_inv;
-        if (g_full_block) { J[7] += s.d_v_l * dx_inv; J[11] += s.d_u_b * dx_inv; }
+        r[1] = fma(s.res, dx_inv, r[1]);
+        J[6] = fma(s.d_u_b, dx_inv, J[6]);
+        J[9] = fma(s.d_eta_H * dH_bl, dx_inv, J[9]);
+        r[2] = fma(s.res, dx_inv, r[2]);
+        J[12] = fma(s.d_v_l, dx_inv, J[12]);
+        J[14] = fma(s.d_eta_H * dH_bl, dx_inv, J[14]);
+        if (g_full_block) { J[7] = fma(s.d_v_l, dx_inv, J[7]); J[11] = fma(s.d_u_b, dx_inv, J[11]); }
     }
     /* bottom-left vertex (i+1, j) */
     {
@@ -1079,13 +1088,13 @@ static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, 
                                 CE(H, i + 1, j - 1, ny, nx), CE(H, i + 1, j, ny, nx));
         double dH_tr = 0.25 * e_c;
         SigS s = sigma_xy(u_l, VF(u, i + 1, j, ny, nx), HF(v, i + 1, j - 1, ny, nx), v_b, eH, dx_inv, i + 1, j, ny, nx);
-        r[0] -= s.res * dx_inv;
-        J[0] -= s.d_u_t * dx_inv;
-        J[4] -= s.d_eta_H * dH_tr * dx_inv;
-        r[3] -= s.res * dx_inv;
-        J[18] -= s.d_v_r * dx_inv;
-        J[19] -= s.d_eta_H * dH_tr * dx_inv;
-        if (g_full_block) { J[3] -= s.d_v_r * dx_inv; J[15] -= s.d_u_t * dx_inv; }
+        r[0] = fma(-s.res, dx_inv, r[0]);
+        J[0] = fma(-s.d_u_t, dx_inv, J[0]);
+        J[4] = fma(-(s.d_eta_H * dH_tr), dx_inv, J[4]);
+        r[3] = fma(-s.res, dx_inv, r[3]);
+        J[18] = fma(-s.d_v_r, dx_inv, J[18]);
+        J[19] = fma(-(s.d_eta_H * dH_tr), dx_inv, J[19]);
+        if (g_full_block) { J[3] = fma(-s.d_v_r, dx_inv, J[3]); J[15] = fma(-s.d_u_t, dx_inv, J[15]); }
     }
     /* bottom-right vertex (i+1, j+1) */
     {
@@ -1095,13 +1104,13 @@ static void vanka_assemble(const VCtx *c, int i, int j, double u_l, double u_r, 
                                 CE(H, i + 1, j, ny, nx), CE(H, i + 1, j + 1, ny, nx));
         double dH_tl = 0.25 * e_c;
         SigS s = sigma_xy(u_r, VF(u, i + 1, j + 1, ny, nx), v_b, HF(v, i + 1, j + 1, ny, nx), eH, dx_inv, i + 1, j + 1, ny, nx);
-        r[1] -= s.res * dx_inv;
-        J[6] -= s.d_u_t * dx_inv;
-        J[9] -= s.d_eta_H * dH_tl * dx_inv;
-        r[3] += s.res * dx_inv;
-        J[18] += s.d_v_l * dx_inv;
-        J[19] += s.d_eta_H * dH_tl * dx_inv;
-        if (g_full_block) { J[8] -= s.d_v_l * dx_inv; J[16] += s.d_u_t * dx_inv; }
+        r[1] = fma(-s.res, dx_inv, r[1]);
+        J[6] = fma(-s.d_u_t, dx_inv, J[6]);
+        J[9] = fma(-(s.d_eta_H * dH_tl), dx_inv, J[9]);
+        r[3] = fma(s.res, dx_inv, r[3]);
+        J[18] = fma(s.d_v_l, dx_inv, J[18]);
+        J[19] = fma(s.d_eta_H * dH_tl, dx_inv, J[19]);
+        if (g_full_block) { J[8] = fma(-s.d_v_l, dx_inv, J[8]); J[16] = fma(s.d_u_t, dx_inv, J[16]); }
     }
     /* basal drag */
     {
