@@ -9,11 +9,11 @@ gl_derivatives off.
 
 # one local Newton iteration of one cell: 5x5 block assembly (including the
 # centre cell's grounding sigmoid) + LU solve + update
-FLOPS_NEWTON_ITER = 739
+FLOPS_NEWTON_ITER = 742
 # viscosity of one cell (computed once per sweep)
-FLOPS_ETA = 122
+FLOPS_ETA = 128
 # per cell and sweep: grounding sigmoids of the 4 edge neighbours
-FLOPS_CELL_SETUP = 152
+FLOPS_CELL_SETUP = 164
 NEWTON_ITERS = 10
 
 PEAK_HBM_GBS = 8000.0        # MI355X HBM3E
