@@ -60,11 +60,10 @@ def parse():
 # WRITE_SIZE (one run each) and calibrated in the same program on kernels of
 # known traffic: a 256 MiB torch.sum reads 131,134 KiB of FETCH_SIZE (bytes =
 # 2048 x FETCH_SIZE) and a 256 MiB fill writes 262,144 KiB of WRITE_SIZE
-# (bytes = 1024 x WRITE_SIZE).  Vanka sweep + apply pass: FETCH 13,540.8 + 8,713.5,
-# WRITE 11,168.3 + 6,248 (the sweep's register spills included); residual
-# (cell tiles): FETCH 14,133.5, WRITE 6,248.
-TRAFFIC_512 = {"vanka": (2 * (13540.8 + 8713.5) + 11168.3 + 6248.0) * 1024,
-               "residual": (2 * 14133.5 + 6248.0) * 1024}
+# (bytes = 1024 x WRITE_SIZE).  Vanka sweep + apply pass: FETCH 13,204.2 + 8,708.5,
+# WRITE 10,240 + 6,248; residual (cell tiles): FETCH 14,133.8, WRITE 6,248.
+TRAFFIC_512 = {"vanka": (2 * (13204.2 + 8708.5) + 10240.0 + 6248.0) * 1024,
+               "residual": (2 * 14133.8 + 6248.0) * 1024}
 
 
 def kernel_rooflines(ph, reps=20):
