@@ -90,9 +90,22 @@ def test_residual_bitwise(shape, gl):
     assert np.array_equal(r.cpu().numpy(), refF - sub)
 
 
-@pytest.mark.parametrize("shape", [(9, 12), (40, 70)])
-def test_vanka_sweep_bitwise(shape):
+@pytest.fixture
+def row_split():
+    """Set the row-split cell limit for one test (0: every level sweeps with
+    k_vanka's tiles); restores the library default afterwards."""
+    yield lambda cells: _lib.call("glide_set_row_split", cells)
+    _lib.call("glide_set_row_split", 128 * 128)
+
+
+# (9, 12) and (40, 70) sweep row-split by default and with k_vanka's
+# boundary and interior tiles when the split is off; 131 x 150 is above the
+# split limit, so its interior tiles take the specialised assembly.
+@pytest.mark.parametrize("split", [True, False])
+@pytest.mark.parametrize("shape", [(9, 12), (40, 70), (131, 150)])
+def test_vanka_sweep_bitwise(shape, split, row_split):
     ny, nx = shape
+    row_split(128 * 128 if split else 0)
     s = random_state(ny=ny, nx=nx, seed=3 * ny + nx, vel=100.0)
     prm = O.params()
     ref = O.vanka_sweep(ny, nx, s["dx"], s["dt"], prm, s["U"], s["f"], s["bed"], s["B"], s["beta"], s["gamma"])
@@ -105,7 +118,9 @@ def test_vanka_sweep_bitwise(shape):
     assert np.array_equal(out, ref), np.abs(out - ref).max()
 
 
-def test_local_activation_and_local_sweep_bitwise():
+@pytest.mark.parametrize("split", [True, False])
+def test_local_activation_and_local_sweep_bitwise(split, row_split):
+    row_split(128 * 128 if split else 0)
     ny, nx = 48, 60
     s = random_state(ny=ny, nx=nx, seed=77, vel=50.0)
     prm = O.params()
