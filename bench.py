@@ -453,7 +453,7 @@ def cpu_baseline_tol(seed: int, vcycles: int, threads: int, dt=DT):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU is out of scope)"}))
+        print(json.dumps({"error": "bench.py measures one GPU; --gpus must be 1 (multi-GPU runs are not part of this benchmark)"}))
         sys.exit(2)
     import numpy as np
     import torch

@@ -5,7 +5,7 @@ maps the cell fields of a domain (H, the bed as flotation margin, |grad s|,
 log beta, B and the signed surface slopes) to the cell-centred (u, v) and
 the thickness change H - H_prev of a forward step; the prediction, mapped to
 the MAC faces and to H = max(H_prev + dH, gamma), replaces the cold initial
-state of the step.  Plain FASCD remains the solver and the judge: the step iterates V-cycles until the reference's residual norm
+state of the step.  Plain FASCD remains the solver and the arbiter: the step iterates V-cycles until the reference's residual norm
 (glide/physics.py:176-203) drops below ``tol`` times the COLD start's initial
 norm, and if the predicted state's initial residual exceeds the cold one the
 prediction is discarded (fallback to the cold start).

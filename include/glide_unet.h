@@ -4,7 +4,7 @@
  * The reference has no learned component: these entry points serve a small
  * convolutional U-Net whose predicted (u, v) seeds the FASCD V-cycles of a
  * forward step (the solve of glide/physics.py:141-209), with plain FASCD as
- * the fallback and the judge of the residual.
+ * the fallback and the arbiter of the residual.
  *
  * Tensors: activations BF16 (raw uint16 bits, round-to-nearest-even), NHWC,
  * every channel count a positive multiple of 32; 3x3 weights [9][Cout][Cin]

@@ -16,7 +16,7 @@ from tests.helpers import random_state
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
-if not torch.cuda.is_available():  # pragma: no cover - collected only on the GPU box
+if not torch.cuda.is_available():  # pragma: no cover - collected only where a GPU is present
     pytest.skip("no GPU", allow_module_level=True)
 
 from glide_amd import _lib  # noqa: E402
