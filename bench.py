@@ -51,6 +51,8 @@ def parse():
                     help="only the cap_512 3-V-cycle steps (for profiling the reference schedule)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the kernel microbenchmarks (for profiling them alone)")
+    ap.add_argument("--inversion-only", action="store_true", help="only the beta-inversion gradient path")
+    ap.add_argument("--no-inversion", action="store_true", help="skip the beta-inversion gradient path")
     ap.add_argument("--cpu-report", action="store_true",
                     help="only the CPU oracle: ms per V-cycle and per step of configs 0 and 1, one thread and all")
     return ap.parse_args()
@@ -429,6 +431,64 @@ def reference_schedule_bench(dt=10.0, steps=3):
             "ms_per_step": round(ms, 3), "rel_residual_after_step": [float(f"{x:.3e}") for x in ph.last_history]}
 
 
+def inversion_bench(n=512, lbfgs_iters=8):
+    """The adjoint inversion's gradient path at 512^2: a grounded slab
+    (synthetic.ice_slab, seed 2) observed through a seeded truth beta with 2%
+    noise, L2 misfit plus 1e-3 Tikhonov on log beta.  Reports one adjoint
+    V-cycle, one objective + gradient evaluation when converged (forward and
+    adjoint to 1e-10) and on the reference's fixed schedule (5 forward
+    V-cycles, 1 adjoint V-cycle), and a short L-BFGS run."""
+    import numpy as np
+    import torch
+
+    from glide_amd import synthetic as S
+    from glide_amd.inversion import BetaInversion
+    from glide_amd.physics import IcePhysics
+
+    w = S.ice_slab(n, n, 1500.0, seed=2)
+    truth = S.ice_slab(n, n, 1500.0, seed=3)["beta"]
+    ph = IcePhysics(n, n, w["dx"], n_levels=5, gl_derivatives=False)
+    ph.set_geometry(w["bed"], w["H"])
+    ph.set_parameters(B=w["B"], beta=truth, smb=w["smb"])
+    ph.forward(10.0, tol=1e-12, max_vcycles=60, update_geometry=False)
+    rng = np.random.default_rng(3)
+    u_obs = ph.grid.u.cpu().numpy() * (1 + 0.02 * rng.standard_normal((n, n + 1)))
+    v_obs = ph.grid.v.cpu().numpy() * (1 + 0.02 * rng.standard_normal((n + 1, n)))
+    x0 = np.full(n * n, np.log(float(np.mean(truth))))
+    out = {"workload": f"beta inversion: {n}x{n} grounded slab, dx=1500 m, 5 levels, dt=10 a, L2 misfit + "
+                       "1e-3 Tikhonov on log beta, observations = truth solve + 2% noise"}
+    modes = (("converged", dict(mode="l2", fwd_tol=1e-10, adj_tol=1e-10)), ("reference_schedule", dict(mode="reference")))
+    for name, kw in modes:
+        inv = BetaInversion(ph, u_obs, v_obs, w["H"], dt=10.0, alpha=1e-3, **kw)
+        inv.objective(x0)
+        reps = 3
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            inv.objective(x0)
+        torch.cuda.synchronize()
+        out[f"{name}_ms_per_evaluation"] = round((time.perf_counter() - t0) / reps * 1e3, 2)
+        if name == "converged":
+            out["converged_vcycles"] = {"forward": len(ph.last_history), "adjoint": len(ph.last_adj_history)}
+    reps = 20
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ph.adjoint_vcycle()
+    torch.cuda.synchronize()
+    out["ms_per_adjoint_vcycle"] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+    inv = BetaInversion(ph, u_obs, v_obs, w["H"], dt=10.0, alpha=1e-3, mode="l2", fwd_tol=1e-10, adj_tol=1e-10)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = inv.run(x0, maxiter=lbfgs_iters)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    out["lbfgs"] = {"iterations": int(res.nit), "evaluations": len(inv.history),
+                    "ms_per_iteration": round(t / max(1, res.nit) * 1e3, 1),
+                    "misfit_ratio": float(f"{inv.history[-1]['misfit'] / inv.history[0]['misfit']:.4e}")}
+    return out
+
+
 def cpu_baseline_tol(seed: int, vcycles: int, threads: int, dt=DT):
     """The CPU oracle (OpenMP) on the headline workload: one V-cycle of the
     512^2 sheet from the cold start timed, times the plain V-cycle count."""
@@ -463,6 +523,9 @@ def main():
         return
     if args.cpu_report:
         print(json.dumps(cpu_report(args.cpu_threads or 16)))
+        return
+    if args.inversion_only:
+        print(json.dumps(inversion_bench()))
         return
     if args.kernels_only:
         from glide_amd import synthetic as S
@@ -512,6 +575,8 @@ def main():
         line["unet_roofline"] = unet_roofline(net)
     line["learned_batched"] = learned_batched_bench(net, [r["seed"] for r in rows])
     line["reference_schedule"] = reference_schedule_bench()
+    if not args.no_inversion:
+        line["inversion"] = inversion_bench()
     if not args.no_ensemble:
         line["ensemble"] = ensemble_bench()
     if not args.no_cpu:

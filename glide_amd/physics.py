@@ -25,7 +25,10 @@ G = 9.81
 def _as_f64(x, device) -> torch.Tensor:
     if isinstance(x, torch.Tensor):
         return x.to(device=device, dtype=torch.float64)
-    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=device)
+    a = np.asarray(x, dtype=np.float64)
+    if not a.flags.writeable:   # read-only input (a view of a constant, a memory map): copy, never alias it
+        a = a.copy()
+    return torch.as_tensor(a, device=device)
 
 
 class IcePhysics:
