@@ -510,10 +510,15 @@ __global__ __launch_bounds__(256) void k_norm_final(int n, const double *__restr
 // ===========================================================================
 // Linearisation: JVP and gather-form VJP
 // ===========================================================================
-__device__ __forceinline__ void fill_eta_dual_tile(Dual (*eta_s)[TX + 2], int i0, int j0, const double *__restrict__ u,
-                                                   const double *__restrict__ v, const double *__restrict__ du,
-                                                   const double *__restrict__ dv, const double *__restrict__ B,
-                                                   const Params &p, double dx, int ny, int nx)
+// Dual viscosities and grounding sigmoids of the cells [i0-1, i0+TY] x
+// [j0-1, j0+TX] into LDS.  The sigmoid of a cell outside the domain is that
+// of its clamped neighbour, as every clamped read of bed and H gives it.
+__device__ __forceinline__ void fill_eta_dual_tile(Dual (*eta_s)[TX + 2], Ground (*g_s)[TX + 2], int i0, int j0,
+                                                   const double *__restrict__ u, const double *__restrict__ v,
+                                                   const double *__restrict__ du, const double *__restrict__ dv,
+                                                   const double *__restrict__ H, const double *__restrict__ bed,
+                                                   const double *__restrict__ B, const Params &p, double dx, int ny,
+                                                   int nx)
 {
     int tid = threadIdx.y * TX + threadIdx.x;
     for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
@@ -521,13 +526,15 @@ __device__ __forceinline__ void fill_eta_dual_tile(Dual (*eta_s)[TX + 2], int i0
         int ci = i0 - 1 + a, cj = j0 - 1 + b;
         eta_s[a][b] = (ci >= 0 && ci < ny && cj >= 0 && cj < nx) ? eta_cell_dual(ci, cj, u, v, du, dv, B, p, dx, ny, nx)
                                                                  : Dual{0.0, 0.0};
+        g_s[a][b] = ground(ld_c(bed, ci, cj, ny, nx), ld_c(H, ci, cj, ny, nx), p);
     }
 }
 
 // d/dt of the u-row R_u(i, j) along (du, dv, dH); dH == NULL means 0
 // (the u-face part of glide/cuda/residuals.cu:378-499)
-template <class EF>
+template <class EF, class GF>
 __device__ __forceinline__ double jvp_u_row(int i, int j, int ny, int nx, double dx_inv, const Params &p, const EF &E,
+                                            const GF &G,
                                             const double *u, const double *v, const double *H, const double *du,
                                             const double *dv, const double *dH, const double *bed,
                                             const double *beta)
@@ -570,8 +577,8 @@ __device__ __forceinline__ double jvp_u_row(int i, int j, int ny, int nx, double
     }
     {
         Dual ul = Ud(i, j), Hl = Hd(i, j - 1), Hc = Hd(i, j);
-        TauB t = tau_b(ul.v, Hl.v, Hc.v, ld_c(bed, i, j - 1, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i, j - 1, ny, nx),
-                       ld_c(beta, i, j, ny, nx), p);
+        const Ground gl = G(i, j - 1), gc = G(i, j);
+        TauB t = tau_b_g(ul.v, gl.g, gc.g, gl.gd, gc.gd, ld_c(beta, i, j - 1, ny, nx), ld_c(beta, i, j, ny, nx), p);
         d += t.d_u * ul.d + t.d_H_l * Hl.d + t.d_H_r * Hc.d;
     }
     {
@@ -583,8 +590,9 @@ __device__ __forceinline__ double jvp_u_row(int i, int j, int ny, int nx, double
 }
 
 // v-row R_v(i, j) (glide/cuda/residuals.cu:501-576)
-template <class EF>
+template <class EF, class GF>
 __device__ __forceinline__ double jvp_v_row(int i, int j, int ny, int nx, double dx_inv, const Params &p, const EF &E,
+                                            const GF &G,
                                             const double *u, const double *v, const double *H, const double *du,
                                             const double *dv, const double *dH, const double *bed,
                                             const double *beta)
@@ -627,8 +635,8 @@ __device__ __forceinline__ double jvp_v_row(int i, int j, int ny, int nx, double
     }
     {
         Dual vt = Vd(i, j), Ht = Hd(i - 1, j), Hc = Hd(i, j);
-        TauB t = tau_b(vt.v, Ht.v, Hc.v, ld_c(bed, i - 1, j, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i - 1, j, ny, nx),
-                       ld_c(beta, i, j, ny, nx), p);
+        const Ground gt = G(i - 1, j), gc = G(i, j);
+        TauB t = tau_b_g(vt.v, gt.g, gc.g, gt.gd, gc.gd, ld_c(beta, i - 1, j, ny, nx), ld_c(beta, i, j, ny, nx), p);
         d += t.d_u * vt.d + t.d_H_l * Ht.d + t.d_H_r * Hc.d;
     }
     {
@@ -645,20 +653,22 @@ __global__ __launch_bounds__(NT) void k_jvp(GlideGeom g, Params p, double *__res
                                             const double *__restrict__ mask)
 {
     __shared__ Dual eta_s[TY + 2][TX + 2];
+    __shared__ Ground g_s[TY + 2][TX + 2];
     const int ny = g.ny, nx = g.nx;
     const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
     const double *u = U, *v = U + nu, *H = U + nu + nv, *du = dU, *dv = dU + nu, *dH = dU + nu + nv;
     const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
-    fill_eta_dual_tile(eta_s, i0, j0, u, v, du, dv, B, p, g.dx, ny, nx);
+    fill_eta_dual_tile(eta_s, g_s, i0, j0, u, v, du, dv, H, bed, B, p, g.dx, ny, nx);
     __syncthreads();
     const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
     if (i > ny || j > nx) return;
     auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
+    auto G = [&](int ci, int cj) { return g_s[ci - i0 + 1][cj - j0 + 1]; };
     const double dx_inv = 1.0 / g.dx, dt = g.dt;
     if (i < ny && j < nx) {   // glide/cuda/residuals.cu:343-375
         Dual H_c = {ld_c(H, i, j, ny, nx), ld_c(dH, i, j, ny, nx)};
         double d = H_c.d / dt;
-        Calv jc = calving(H_c.v, ld_c(bed, i, j, ny, nx), p);
+        Calv jc = calving_g(H_c.v, G(i, j), p);
         d -= jc.d_H * H_c.d;
         double Hl = ld_c(H, i, j - 1, ny, nx), dHl = ld_c(dH, i, j - 1, ny, nx);
         double ul = ld_u(u, i, j, ny, nx), dul = ld_u(du, i, j, ny, nx);
@@ -679,8 +689,8 @@ __global__ __launch_bounds__(NT) void k_jvp(GlideGeom g, Params p, double *__res
         double m = ld_c(mask, i, j, ny, nx);
         jo[nu + nv + (size_t)i * nx + j] = (1.0 - m) * d + m * H_c.d;
     }
-    if (i < ny) jo[(size_t)i * (nx + 1) + j] = jvp_u_row(i, j, ny, nx, dx_inv, p, E, u, v, H, du, dv, dH, bed, beta);
-    if (j < nx) jo[nu + (size_t)i * nx + j] = jvp_v_row(i, j, ny, nx, dx_inv, p, E, u, v, H, du, dv, dH, bed, beta);
+    if (i < ny) jo[(size_t)i * (nx + 1) + j] = jvp_u_row(i, j, ny, nx, dx_inv, p, E, G, u, v, H, du, dv, dH, bed, beta);
+    if (j < nx) jo[nu + (size_t)i * nx + j] = jvp_v_row(i, j, ny, nx, dx_inv, p, E, G, u, v, H, du, dv, dH, bed, beta);
 }
 
 // shear-rate derivative factor 2*eps_xy at vertex (a, b), 0 on boundary vertices
@@ -700,15 +710,17 @@ __global__ __launch_bounds__(NT) void k_vjp(GlideGeom g, Params p, double *__res
                                             const double *__restrict__ beta, const double *__restrict__ mask)
 {
     __shared__ Dual eta_s[TY + 2][TX + 2];
+    __shared__ Ground g_s[TY + 2][TX + 2];
     const int ny = g.ny, nx = g.nx;
     const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
     const double *u = U, *v = U + nu, *H = U + nu + nv, *lu = lam, *lv = lam + nu, *lH = lam + nu + nv;
     const int i0 = blockIdx.y * TY, j0 = blockIdx.x * TX;
-    fill_eta_dual_tile(eta_s, i0, j0, u, v, lu, lv, B, p, g.dx, ny, nx);
+    fill_eta_dual_tile(eta_s, g_s, i0, j0, u, v, lu, lv, H, bed, B, p, g.dx, ny, nx);
     __syncthreads();
     const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
     if (i > ny || j > nx) return;
     auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 1][cj - j0 + 1]; };
+    auto G = [&](int ci, int cj) { return g_s[ci - i0 + 1][cj - j0 + 1]; };
     const double dx_inv = 1.0 / g.dx, dt = g.dt;
     // lambda_H of the mass row of cell (a, b) with the active-set mask applied
     auto lm = [&](int a, int b) { return ld_c(lH, a, b, ny, nx) * (1.0 - ld_c(mask, a, b, ny, nx)); };
@@ -719,7 +731,7 @@ __global__ __launch_bounds__(NT) void k_vjp(GlideGeom g, Params p, double *__res
         double acc = 0.0;
         if (j > 0) acc += lm(i, j - 1) * q.d_u * dx_inv;
         if (j < nx) acc += -lm(i, j) * q.d_u * dx_inv;
-        acc += jvp_u_row(i, j, ny, nx, dx_inv, p, E, u, v, H, lu, lv, (const double *)nullptr, bed, beta);
+        acc += jvp_u_row(i, j, ny, nx, dx_inv, p, E, G, u, v, H, lu, lv, (const double *)nullptr, bed, beta);
         emit((size_t)i * (nx + 1) + j, acc);
     }
     if (j < nx) {   // v face (i, j)
@@ -727,7 +739,7 @@ __global__ __launch_bounds__(NT) void k_vjp(GlideGeom g, Params p, double *__res
         double acc = 0.0;
         if (i > 0) acc += -lm(i - 1, j) * q.d_u * dx_inv;
         if (i < ny) acc += lm(i, j) * q.d_u * dx_inv;
-        acc += jvp_v_row(i, j, ny, nx, dx_inv, p, E, u, v, H, lu, lv, (const double *)nullptr, bed, beta);
+        acc += jvp_v_row(i, j, ny, nx, dx_inv, p, E, G, u, v, H, lu, lv, (const double *)nullptr, bed, beta);
         emit(nu + (size_t)i * nx + j, acc);
     }
     if (i < ny && j < nx) {   // cell (i, j)
@@ -739,7 +751,7 @@ __global__ __launch_bounds__(NT) void k_vjp(GlideGeom g, Params p, double *__res
         Flux qxr = flux_x(ld_u(u, i, j + 1, ny, nx), Hc, Hr, j + 1, nx);
         Flux qyt = flux_y(ld_v(v, i, j, ny, nx), Ht, Hc, i, ny);
         Flux qyb = flux_y(ld_v(v, i + 1, j, ny, nx), Hc, Hb, i + 1, ny);
-        Calv jc = calving(Hc, ld_c(bed, i, j, ny, nx), p);
+        Calv jc = calving_g(Hc, G(i, j), p);
         // own mass row (residuals.cu:648-690)
         double acc = lc / dt;
         acc += -lc * jc.d_H;
@@ -775,29 +787,29 @@ __global__ __launch_bounds__(NT) void k_vjp(GlideGeom g, Params p, double *__res
         }
         // basal drag (grounding-line derivatives) and driving stress on the four faces
         {
-            TauB t = tau_b(ul, Hl, Hc, ld_c(bed, i, j - 1, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i, j - 1, ny, nx),
-                           ld_c(beta, i, j, ny, nx), p);
+            const Ground ga = G(i, j - 1), gb = G(i, j);
+            TauB t = tau_b_g(ul, ga.g, gb.g, ga.gd, gb.gd, ld_c(beta, i, j - 1, ny, nx), ld_c(beta, i, j, ny, nx), p);
             TauD d = tau_dx(Hl, Hc, ld_c(bed, i, j - 1, ny, nx), ld_c(bed, i, j, ny, nx), dx_inv, j, nx);
             acc += lu_l * t.d_H_r;
             acc += -lu_l * d.d_H_r;
         }
         {
-            TauB t = tau_b(ur, Hc, Hr, ld_c(bed, i, j, ny, nx), ld_c(bed, i, j + 1, ny, nx), ld_c(beta, i, j, ny, nx),
-                           ld_c(beta, i, j + 1, ny, nx), p);
+            const Ground ga = G(i, j), gb = G(i, j + 1);
+            TauB t = tau_b_g(ur, ga.g, gb.g, ga.gd, gb.gd, ld_c(beta, i, j, ny, nx), ld_c(beta, i, j + 1, ny, nx), p);
             TauD d = tau_dx(Hc, Hr, ld_c(bed, i, j, ny, nx), ld_c(bed, i, j + 1, ny, nx), dx_inv, j + 1, nx);
             acc += lu_r * t.d_H_l;
             acc += -lu_r * d.d_H_l;
         }
         {
-            TauB t = tau_b(vt, Ht, Hc, ld_c(bed, i - 1, j, ny, nx), ld_c(bed, i, j, ny, nx), ld_c(beta, i - 1, j, ny, nx),
-                           ld_c(beta, i, j, ny, nx), p);
+            const Ground ga = G(i - 1, j), gb = G(i, j);
+            TauB t = tau_b_g(vt, ga.g, gb.g, ga.gd, gb.gd, ld_c(beta, i - 1, j, ny, nx), ld_c(beta, i, j, ny, nx), p);
             TauD d = tau_dy(Ht, Hc, ld_c(bed, i - 1, j, ny, nx), ld_c(bed, i, j, ny, nx), dx_inv, i, ny);
             acc += lv_t * t.d_H_r;
             acc += -lv_t * d.d_H_r;
         }
         {
-            TauB t = tau_b(vb, Hc, Hb, ld_c(bed, i, j, ny, nx), ld_c(bed, i + 1, j, ny, nx), ld_c(beta, i, j, ny, nx),
-                           ld_c(beta, i + 1, j, ny, nx), p);
+            const Ground ga = G(i, j), gb = G(i + 1, j);
+            TauB t = tau_b_g(vb, ga.g, gb.g, ga.gd, gb.gd, ld_c(beta, i, j, ny, nx), ld_c(beta, i + 1, j, ny, nx), p);
             TauD d = tau_dy(Hc, Hb, ld_c(bed, i, j, ny, nx), ld_c(bed, i + 1, j, ny, nx), dx_inv, i + 1, ny);
             acc += lv_b * t.d_H_l;
             acc += -lv_b * d.d_H_l;
