@@ -53,6 +53,8 @@ def parse():
                     help="only the kernel microbenchmarks (for profiling them alone)")
     ap.add_argument("--inversion-only", action="store_true", help="only the beta-inversion gradient path")
     ap.add_argument("--no-inversion", action="store_true", help="skip the beta-inversion gradient path")
+    ap.add_argument("--no-1024", action="store_true", help="skip the 1024^2 warm-start solves")
+    ap.add_argument("--only-1024", action="store_true", help="only the 1024^2 warm-start solves")
     ap.add_argument("--cpu-report", action="store_true",
                     help="only the CPU oracle: ms per V-cycle and per step of configs 0 and 1, one thread and all")
     return ap.parse_args()
@@ -357,6 +359,38 @@ def to_tolerance_bench(steps: int, warmup: int, dt=DT):
     return rows, last, net
 
 
+def learned_1024_bench(net, n_seeds=2, max_vcycles=80, dt=DT):
+    """The warm start at 1024^2 (dx = 1000 m, 6 levels): one dt = 1 a step
+    to 1e-8 from the cold start and from the network's prediction (with the
+    headline's fallback rule), on the first test seeds; at most max_vcycles
+    V-cycles each."""
+    import torch
+
+    from glide_amd.train_warmstart import TEST_SEED0, member
+    from glide_amd.warmstart import WarmStart
+
+    rows = []
+    for seed in range(TEST_SEED0, TEST_SEED0 + n_seeds):
+        row = {"seed": seed}
+        for name, ws, use in (("cold", WarmStart(None), False), ("learned", WarmStart(net), True)):
+            m = member(seed, 1024, dx=1000.0, n_levels=6)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = ws.solve(m, dt, tol=1e-8, max_vcycles=max_vcycles, use_net=use)
+            torch.cuda.synchronize()
+            row[name] = {"ms": round((time.perf_counter() - t0) * 1e3, 1), "vcycles": r["vcycles"],
+                         "converged": bool(r["converged"]),
+                         "final_rel_residual": float(f"{r['history'][-1]:.3e}") if r["history"] else None}
+            if use:
+                row[name]["r0_warm_over_cold"] = round(r["r0_warm"] / r["r0_cold"], 3) if r["r0_warm"] else None
+                row[name]["fallback"] = bool(r["fallback"])
+            del m
+            torch.cuda.empty_cache()
+        rows.append(row)
+    return {"workload": f"1024x1024, dx=1000 m, 6 levels, dt={dt:g} a, one forward step to 1e-8 (at most "
+                        f"{max_vcycles} V-cycles), cold start vs learned warm start", "seeds": rows}
+
+
 def learned_batched_bench(net, seeds, n_members=16, dt=DT):
     """The headline seeds (each repeated to fill 16 members) as one ensemble:
     plain batched FASCD vs batched network inference + batched FASCD, both to
@@ -527,6 +561,10 @@ def main():
     if args.inversion_only:
         print(json.dumps(inversion_bench()))
         return
+    if args.only_1024:
+        from glide_amd.warmstart import UNet
+        print(json.dumps(learned_1024_bench(UNet.load(WEIGHTS))))
+        return
     if args.kernels_only:
         from glide_amd import synthetic as S
         from glide_amd.physics import IcePhysics
@@ -573,6 +611,8 @@ def main():
     }
     if learned:
         line["unet_roofline"] = unet_roofline(net)
+        if not args.no_1024:
+            line["learned_1024"] = learned_1024_bench(net)
     line["learned_batched"] = learned_batched_bench(net, [r["seed"] for r in rows])
     line["reference_schedule"] = reference_schedule_bench()
     if not args.no_inversion:
