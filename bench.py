@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--inversion-only", action="store_true", help="only the beta-inversion gradient path")
     ap.add_argument("--no-inversion", action="store_true", help="skip the beta-inversion gradient path")
     ap.add_argument("--no-1024", action="store_true", help="skip the 1024^2 warm-start solves")
+    ap.add_argument("--ensemble-only", action="store_true", help="only the 64-member batched ensemble")
     ap.add_argument("--only-1024", action="store_true", help="only the 1024^2 warm-start solves")
     ap.add_argument("--cpu-report", action="store_true",
                     help="only the CPU oracle: ms per V-cycle and per step of configs 0 and 1, one thread and all")
@@ -560,6 +561,9 @@ def main():
         return
     if args.inversion_only:
         print(json.dumps(inversion_bench()))
+        return
+    if args.ensemble_only:
+        print(json.dumps(ensemble_bench()))
         return
     if args.only_1024:
         from glide_amd.warmstart import UNet
