@@ -44,7 +44,16 @@ class GlideLevel(ctypes.Structure):
 
 class GlideSchedule(ctypes.Structure):
     _fields_ = [("pre", c_int), ("post", c_int), ("coarse", c_int), ("local_passes", c_int),
-                ("local_sweeps", c_int), ("n_newton", c_int), ("adj_sweeps", c_int), ("adj_coarse", c_int)]
+                ("local_sweeps", c_int), ("n_newton", c_int), ("adj_sweeps", c_int), ("adj_coarse", c_int),
+                ("boundary_weight", c_double), ("local_skip", c_int)]
+
+
+def schedule(pre=10, post=10, coarse=200, local_passes=10, local_sweeps=10, n_newton=10, adj_sweeps=10,
+             adj_coarse=100, boundary_weight=1.0, local_skip=True) -> GlideSchedule:
+    """A GlideSchedule; the defaults are the reference's sweep counts and
+    residual restriction, with the idle-local-pass check on."""
+    return GlideSchedule(int(pre), int(post), int(coarse), int(local_passes), int(local_sweeps), int(n_newton),
+                         int(adj_sweeps), int(adj_coarse), float(boundary_weight), int(bool(local_skip)))
 
 
 class GlideVtiArray(ctypes.Structure):
@@ -74,7 +83,11 @@ SIGNATURES = {
     "glide_prolong_H": (c_int, [DP, DP, c_int, c_int, c_void_p]),
     "glide_restrict_parameters": (c_int, [POINTER(GlideLevel), c_int, c_void_p]),
     "glide_propagate_geometry": (c_int, [POINTER(GlideLevel), c_int, c_double, c_void_p]),
-    "glide_fascd_vcycle": (c_int, [POINTER(GlideLevel), c_int, c_double, POINTER(GlideSchedule), c_void_p]),
+    "glide_fascd_vcycle": (c_int, [POINTER(GlideLevel), c_int, c_double, POINTER(GlideSchedule), c_void_p,
+                                   c_void_p]),
+    "glide_vcycle_work_create": (c_int, [POINTER(c_void_p)]),
+    "glide_vcycle_work_destroy": (c_int, [c_void_p]),
+    "glide_vcycle_work_counts": (c_int, [c_void_p, POINTER(ctypes.c_long), POINTER(ctypes.c_long), c_int]),
     "glide_forward_setup": (c_int, [POINTER(GlideLevel), c_int, c_double, c_void_p]),
     "glide_residual_norm": (c_int, [POINTER(GlideLevel), POINTER(c_double), c_void_p]),
     "glide_adjoint_vcycle": (c_int, [POINTER(GlideLevel), c_int, POINTER(GlideSchedule), c_void_p]),
@@ -120,16 +133,14 @@ SIGNATURES = {
                                             c_void_p]),
     "glide_set_persistence": (c_int, [c_int, c_int]),
     "glide_set_row_split": (c_int, [ctypes.c_long]),
-    "glide_set_local_skip": (c_int, [c_int]),
     "glide_count_nonfinite": (c_int, [DP, ctypes.c_long, c_void_p, POINTER(ctypes.c_ulonglong), c_void_p]),
-    "glide_local_pass_counts": (c_int, [POINTER(ctypes.c_long), POINTER(ctypes.c_long), c_int]),
     "glide_vti_block_bytes": (ctypes.c_size_t, [POINTER(GlideVtiArray), c_int, c_int, c_int]),
     "glide_vti_pack": (c_int, [POINTER(GlideVtiArray), c_int, c_int, c_int, c_int, c_void_p, ctypes.c_size_t,
                                c_void_p]),
     "glide_interp_grid": (c_int, [DP, c_int, c_int, DP, c_int, DP, c_int, DP, c_int, DP, c_int, c_double, DP,
                                   c_void_p]),
     "glide_bedmachine_thickness": (c_int, [DP, DP, ctypes.c_long, c_double, DP, c_void_p]),
-    "glide_last_error": (c_char_p, []),
+    "glide_last_error": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     "glide_version": (c_char_p, []),
 }
 
@@ -149,8 +160,8 @@ def load():
     import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
     for name in ("glide_last_error", "glide_version"):
-        fn = getattr(lib, name)
-        fn.restype, fn.argtypes = SIGNATURES[name]
+        f = getattr(lib, name)
+        f.restype, f.argtypes = SIGNATURES[name]
     _lib = lib
     return lib
 
@@ -168,9 +179,40 @@ def fn(name: str):
     return f
 
 
+def last_error() -> str:
+    """Message of the last failed call on this thread ("" if none)."""
+    f = fn("glide_last_error")
+    n = f(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    f(buf, n + 1)
+    return buf.value.decode(errors="replace")
+
+
 def call(name: str, *args) -> None:
     """Call a status-returning entry point and raise GlideError on failure."""
-    lib = load()
     rc = fn(name)(*args)
     if rc != 0:
-        raise GlideError(name, rc, lib.glide_last_error().decode(errors="replace"))
+        raise GlideError(name, rc, last_error())
+
+
+class VcycleWork:
+    """Owner of a GlideVcycleWork (the single-domain V-cycle's idle-pass
+    check: pinned flag slots, events, pass counters), created for the
+    current device and destroyed with this object."""
+
+    def __init__(self):
+        h = c_void_p()
+        call("glide_vcycle_work_create", ctypes.byref(h))
+        self.handle = h
+
+    def counts(self, reset: bool = False):
+        """(local passes run, local passes skipped) since the last reset."""
+        run, skipped = ctypes.c_long(0), ctypes.c_long(0)
+        call("glide_vcycle_work_counts", self.handle, ctypes.byref(run), ctypes.byref(skipped), int(bool(reset)))
+        return run.value, skipped.value
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and _lib is not None:
+            fn("glide_vcycle_work_destroy")(h)
+            self.handle = None

@@ -13,10 +13,12 @@
 // workgroup maps a global tile index to its entry by binary search.
 //
 // Persistent launches.  A launch has min(tiles, CUs x WGs-per-CU) workgroups
-// (a multiple of 8) that loop over the tiles.  Workgroups are dispatched
-// round-robin over the 8 XCDs of the MI355X, so workgroup b runs on XCD b % 8:
-// each XCD is given one contiguous eighth of the tile list, which keeps
-// neighbouring tiles (shared halo rows) in the same L2.
+// (a multiple of 8) that loop over the tiles (tile_loop, tiles_device.h).
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X, so
+// workgroup b runs on XCD b % 8; the tile list is cut into chunks of up to 8
+// consecutive tiles dealt round-robin to the XCDs, so neighbouring tiles
+// (shared halo rows) meet in one L2 while spatially clustered work (the
+// local smoother's active blocks) still spreads over every XCD.
 //
 // The per-point arithmetic is the single-level kernels' (tiles_device.h),
 // so a member's trajectory in the ensemble is bitwise identical to solving it
@@ -153,11 +155,13 @@ __global__ void kb_set_int(int *__restrict__ x, int v) { *x = v; }
 
 // Element-wise passes on point tiles: point (i, j) owns u[i, j] (i < ny),
 // v[i, j] (j < nx) and the cell (i, j) (both).  Same expressions as the
-// single-level 1-D kernels.
+// single-level 1-D kernels.  arg is thklim, except for OP_RESTRICT_R where it
+// is the weight of the restricted residual's boundary-face rows.
 template <int OP>
 __global__ __launch_bounds__(BTX *PTY) void kb_point(const BEntry *__restrict__ E, const int *__restrict__ pre, int n,
-                                                      const unsigned char *__restrict__ on, double thklim)
+                                                      const unsigned char *__restrict__ on, double arg)
 {
+    const double thklim = arg;
     tile_loop(pre[n], [&](int t) {
         const int e = find_entry(pre, n, t);
         const BEntry &B = E[e];
@@ -232,15 +236,18 @@ __global__ __launch_bounds__(BTX *PTY) void kb_point(const BEntry *__restrict__ 
                 const double *src = OP == OP_RESTRICT_U ? F.U : F.r;
                 double *dst = OP == OP_RESTRICT_U ? A.U : A.r;
                 double *dst2 = OP == OP_RESTRICT_U ? A.w : nullptr;
+                const double bw = OP == OP_RESTRICT_R ? arg : 1.0;
                 if (hu) {
                     double val = 0.5 * (src[(size_t)(2 * i) * (2 * nx + 1) + 2 * j] +
                                         src[(size_t)(2 * i + 1) * (2 * nx + 1) + 2 * j]);
+                    if (j == 0 || j == nx) val *= bw;
                     dst[ku] = val;
                     if (dst2) dst2[ku] = val;
                 }
                 if (hv) {
                     const double *vf = src + nuf;
                     double val = 0.5 * (vf[(size_t)(2 * i) * (2 * nx) + 2 * j] + vf[(size_t)(2 * i) * (2 * nx) + 2 * j + 1]);
+                    if (i == 0 || i == ny) val *= bw;
                     dst[kv] = val;
                     if (dst2) dst2[kv] = val;
                 }
@@ -517,13 +524,13 @@ int b_fascd(const GlideBatch *b, int d, double thklim, const GlideSchedule &sc, 
     BTRY(b_point<OP_FILL_GAMMA>(b, d, thklim, st));
     BTRY(b_point<OP_RESTRICT_U>(b, d, thklim, st));
     BTRY(b_residual(b, d, false, P.shape[BK_RESIDUAL], P.occ[BK_RESIDUAL], st));
-    BTRY(b_point<OP_RESTRICT_R>(b, d, thklim, st));
+    BTRY(b_point<OP_RESTRICT_R>(b, d, sc.boundary_weight, st));
     BTRY(b_residual(b, d, true, P.shape[BK_RESIDUAL_CHILD], P.occ[BK_RESIDUAL_CHILD], st));
     BTRY(b_fascd(b, d + 1, thklim, sc, st));
     BTRY(b_point<OP_FAS_CORRECT>(b, d, thklim, st));
     BTRY(b_point<OP_GAMMA>(b, d, thklim, st));
     BTRY(b_sweeps(b, d, false, sc.post, sc.n_newton, st));
-    const bool skip = glide::skip_idle_local();
+    const bool skip = sc.local_skip != 0;
     for (int pass = 0; pass < sc.local_passes; pass++) {
         BTRY(b_residual(b, d, false, P.shape[BK_RESIDUAL], P.occ[BK_RESIDUAL], st));
         if (skip) {
@@ -731,9 +738,10 @@ int glide_batch_forward_setup(GlideBatch *b, double dt, void *stream)
 int glide_batch_vcycle(GlideBatch *b, double thklim, const GlideSchedule *sched, void *stream)
 {
     if (!b) return bfail(GLIDE_E_NULL, "batch is NULL");
-    GlideSchedule sc = sched ? *sched : GlideSchedule{10, 10, 200, 10, 10, 10, 10, 100};
+    GlideSchedule sc = sched ? *sched : GlideSchedule{10, 10, 200, 10, 10, 10, 10, 100, 1.0, 0};
     if (sc.pre < 0 || sc.post < 0 || sc.coarse < 0 || sc.local_passes < 0 || sc.local_sweeps < 0 || sc.n_newton < 0)
-        return bfail(GLIDE_E_ARG, "negative sweep count");
+        return bfail(GLIDE_E_ARG, "schedule: negative count");
+    if (!std::isfinite(sc.boundary_weight)) return bfail(GLIDE_E_ARG, "schedule: boundary_weight is not finite");
     return b_fascd(b, 0, thklim, sc, (hipStream_t)stream);
 }
 

@@ -10,6 +10,7 @@
 // so no atomics are needed and the result is deterministic).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -58,13 +59,11 @@ inline hipStream_t S(void *s) { return (hipStream_t)s; }
 // residual); set by glide_set_persistence.  The Vanka sweep's 64 gives one
 // tile per workgroup at every level size used here (measured fastest: the
 // hardware then balances clustered local-smoother work); the residual's 4
-// loops each workgroup over several tiles.
-int g_wg_per_cu[2] = {64, 4};
+// loops each workgroup over several tiles.  Launch shapes only: results do
+// not depend on them.
+std::atomic<int> g_wg_per_cu[2] = {{64}, {4}};
 // Levels with at most this many cells use the row-split Vanka sweep.
-long g_row_split_max_cells = 128 * 128;
-// Stop a level's local passes at the first pass that activates no block.
-bool g_skip_idle_local = true;
-long g_local_passes_run = 0, g_local_passes_skipped = 0;
+std::atomic<long> g_row_split_max_cells{128 * 128};
 
 int n_cu()
 {
@@ -94,7 +93,6 @@ inline unsigned blocks1d(size_t n, int bs = 256) { return (unsigned)((n + bs - 1
 
 // error channel shared with the batched entry points (glide_batch.hip)
 int glide::set_error(int code, const char *msg) { return fail(code, msg); }
-bool glide::skip_idle_local() { return g_skip_idle_local; }
 
 // ---------------------------------------------------------------------------
 // Residual r = F(U) - f (- sub)  (glide/cuda/residuals.cu:6-292)
@@ -304,23 +302,27 @@ __global__ void k_set_int(int *__restrict__ x, int v) { *x = v; }
 // ---------------------------------------------------------------------------
 // Transfer operators (glide/cuda/utility.cu)
 // ---------------------------------------------------------------------------
+// bw multiplies the boundary-face rows (I = 0, I = nxc for u; J = 0, J = nyc
+// for v): 1.0 for states and the reference's residual restriction.
 __global__ void k_restrict_u(const double *__restrict__ uf, double *__restrict__ uc, double *__restrict__ uc2,
-                             int nyc, int nxc)
+                             int nyc, int nxc, double bw)
 {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nyc * (nxc + 1)) return;
     int J = idx / (nxc + 1), I = idx - J * (nxc + 1);
     double val = 0.5 * (uf[(size_t)(2 * J) * (2 * nxc + 1) + 2 * I] + uf[(size_t)(2 * J + 1) * (2 * nxc + 1) + 2 * I]);
+    if (I == 0 || I == nxc) val *= bw;
     uc[idx] = val;
     if (uc2) uc2[idx] = val;
 }
 __global__ void k_restrict_v(const double *__restrict__ vf, double *__restrict__ vc, double *__restrict__ vc2,
-                             int nyc, int nxc)
+                             int nyc, int nxc, double bw)
 {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (nyc + 1) * nxc) return;
     int J = idx / nxc, I = idx - J * nxc;
     double val = 0.5 * (vf[(size_t)(2 * J) * (2 * nxc) + 2 * I] + vf[(size_t)(2 * J) * (2 * nxc) + 2 * I + 1]);
+    if (J == 0 || J == nyc) val *= bw;
     vc[idx] = val;
     if (vc2) vc2[idx] = val;
 }
@@ -1009,59 +1011,71 @@ int launch_activation(const GlideGeom &g, const double *r, const double *U, cons
     return check_launch("local activation");
 }
 
-// Host side of the idle-pass check: two pinned slots for the activation
-// flags of consecutive passes and an event after each copy.  Created on
-// first use, once per host thread and device, and kept for the life of the
-// process.
-struct PassCheck {
+}  // namespace
+
+// Host side of the single-domain V-cycle's idle-pass check (include/glide_hip.h):
+// two pinned slots for the activation flags of consecutive passes, an event
+// after each copy, and the pass counters.
+struct GlideVcycleWork {
+    int device = 0;
     int *slot = nullptr;
     hipEvent_t ev[2] = {};
+    bool pending[2] = {false, false};   // copy posted, not yet waited for
+    hipStream_t stream = nullptr;       // stream of the last posted copy
+    long run = 0, skipped = 0;
 };
-constexpr int MAX_DEVICES = 64;
-thread_local PassCheck t_pass_check[MAX_DEVICES];
 
-int pass_check(PassCheck **out)
+namespace {
+
+// Make the workspace ready for a V-cycle on stream st: flag copies still in
+// flight on another stream are waited for, so none can land in a slot after
+// this V-cycle's own copy.
+int work_begin(GlideVcycleWork *w, hipStream_t st)
 {
     int dev = 0;
-    hipError_t ge = hipGetDevice(&dev);
-    if (ge != hipSuccess) return fail((int)ge, std::string("local pass check: ") + hipGetErrorString(ge));
-    if (dev < 0 || dev >= MAX_DEVICES) return fail(GLIDE_E_ARG, "local pass check: device index out of range");
-    PassCheck &c = t_pass_check[dev];
-    if (!c.slot) {
-        hipError_t e = hipHostMalloc((void **)&c.slot, 2 * sizeof(int), hipHostMallocDefault);
-        for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&c.ev[k], hipEventDisableTiming);
-        if (e != hipSuccess) return fail((int)e, std::string("local pass check: ") + hipGetErrorString(e));
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail((int)e, std::string("local pass check: ") + hipGetErrorString(e));
+    if (dev != w->device) return fail(GLIDE_E_ARG, "local pass check: the workspace belongs to another device");
+    if (st != w->stream) {
+        for (int k = 0; k < 2; k++) {
+            if (!w->pending[k]) continue;
+            e = hipEventSynchronize(w->ev[k]);
+            if (e != hipSuccess) return fail((int)e, std::string("local pass check: ") + hipGetErrorString(e));
+            w->pending[k] = false;
+        }
+        w->stream = st;
     }
-    *out = &c;
     return GLIDE_OK;
 }
 
 // Enqueue the copy of a pass's activation flag into its pinned slot.
-int post_flag(PassCheck *c, int k, const int *flag_dev, hipStream_t st)
+int post_flag(GlideVcycleWork *w, int k, const int *flag_dev, hipStream_t st)
 {
-    hipError_t e = hipMemcpyAsync(&c->slot[k], flag_dev, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipEventRecord(c->ev[k], st);
+    hipError_t e = hipMemcpyAsync(&w->slot[k], flag_dev, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(w->ev[k], st);
     if (e != hipSuccess) return fail((int)e, std::string("local pass check: ") + hipGetErrorString(e));
+    w->pending[k] = true;
     return GLIDE_OK;
 }
 
 // Wait for slot k's copy and report whether that activation marked a block.
-int read_flag(PassCheck *c, int k, bool *any)
+int read_flag(GlideVcycleWork *w, int k, bool *any)
 {
-    hipError_t e = hipEventSynchronize(c->ev[k]);
+    hipError_t e = hipEventSynchronize(w->ev[k]);
     if (e != hipSuccess) return fail((int)e, std::string("local pass check: ") + hipGetErrorString(e));
-    *any = c->slot[k] != 0;
+    w->pending[k] = false;
+    *any = w->slot[k] != 0;
     return GLIDE_OK;
 }
 
 int restrict_state(const GlideLevel &F, const GlideLevel &C, const double *src, double *dst, double *dst2,
-                   hipStream_t st)
+                   hipStream_t st, double bw = 1.0)
 {
     const int nyc = C.g.ny, nxc = C.g.nx, nyf = F.g.ny, nxf = F.g.nx;
     const size_t nuf = n_u(nyf, nxf), nvf = n_v(nyf, nxf), nuc = n_u(nyc, nxc), nvc = n_v(nyc, nxc);
-    hipLaunchKernelGGL(k_restrict_u, dim3(blocks1d(nuc)), dim3(256), 0, st, src, dst, dst2, nyc, nxc);
+    hipLaunchKernelGGL(k_restrict_u, dim3(blocks1d(nuc)), dim3(256), 0, st, src, dst, dst2, nyc, nxc, bw);
     hipLaunchKernelGGL(k_restrict_v, dim3(blocks1d(nvc)), dim3(256), 0, st, src + nuf, dst + nuc,
-                       dst2 ? dst2 + nuc : nullptr, nyc, nxc);
+                       dst2 ? dst2 + nuc : nullptr, nyc, nxc, bw);
     hipLaunchKernelGGL(k_restrict_cell, dim3(blocks1d(n_c(nyc, nxc))), dim3(256), 0, st, src + nuf + nvf,
                        dst + nuc + nvc, dst2 ? dst2 + nuc + nvc : nullptr, nyc, nxc);
     return check_launch("restrict state");
@@ -1085,7 +1099,16 @@ int fill(double *a, size_t n, double v, hipStream_t st)
         if (_rc != GLIDE_OK) return _rc; \
     } while (0)
 
-GlideSchedule default_schedule() { return GlideSchedule{10, 10, 200, 10, 10, 10, 10, 100}; }
+GlideSchedule default_schedule() { return GlideSchedule{10, 10, 200, 10, 10, 10, 10, 100, 1.0, 0}; }
+
+int check_schedule(const GlideSchedule &sc)
+{
+    if (sc.pre < 0 || sc.post < 0 || sc.coarse < 0 || sc.local_passes < 0 || sc.local_sweeps < 0 || sc.n_newton < 0 ||
+        sc.adj_sweeps < 0 || sc.adj_coarse < 0)
+        return fail(GLIDE_E_ARG, "schedule: negative count");
+    if (!std::isfinite(sc.boundary_weight)) return fail(GLIDE_E_ARG, "schedule: boundary_weight is not finite");
+    return GLIDE_OK;
+}
 
 int check_levels(const GlideLevel *L, int n)
 {
@@ -1103,7 +1126,8 @@ int check_levels(const GlideLevel *L, int n)
 }
 
 // glide/solver.py:71-157
-int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const GlideSchedule &sc, hipStream_t st)
+int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const GlideSchedule &sc,
+          GlideVcycleWork *work, hipStream_t st)
 {
     const GlideLevel &A = L[k];
     const int ny = A.g.ny, nx = A.g.nx;
@@ -1129,17 +1153,17 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
     TRY(fill(A.gamma, nc, thklim, st));
     TRY(restrict_state(A, C, A.U, C.U, C.w, st));
     TRY(launch_residual(A.g, p, A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma, st));
-    TRY(restrict_state(A, C, A.r, C.r, nullptr, st));
+    TRY(restrict_state(A, C, A.r, C.r, nullptr, st, sc.boundary_weight));
     // f_c = F_c(U_c) - r_c
     TRY(launch_residual(C.g, pc, C.f, C.U, nullptr, C.r, C.bed, C.B, C.beta, C.mask, C.gamma, st));
-    TRY(fascd(L, n, k + 1, false, thklim, sc, st));
+    TRY(fascd(L, n, k + 1, false, thklim, sc, work, st));
     hipLaunchKernelGGL(k_fas_correct, dim3(blocks1d(nt)), dim3(256), 0, st, ny, nx, A.U, A.w, C.U, C.w);
     hipLaunchKernelGGL(k_gamma, dim3(blocks1d(nc)), dim3(256), 0, st, ny, nx, A.gamma, wH, A.chi, nullptr);
     TRY(check_launch("fas correct"));
     TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, nullptr, sc.post, sc.n_newton, A.scratch, st));
     // A pass whose activation marks no block leaves U unchanged (its apply
     // passes add +0.0), so every later pass of this V-cycle recomputes the
-    // same residual and activation and is idle too.  With g_skip_idle_local
+    // same residual and activation and is idle too.  With sc.local_skip
     // the activation also raises an any-block flag (pass k's in int slot
     // k & 1 at the start of the Vanka scratch, which the sweeps rewrite only
     // after the flag has been copied out), and the host stops the level's
@@ -1152,8 +1176,7 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
     //   idle pass found this way has had its (idle) sweeps enqueued.
     // The only difference from running every pass is the sign of zero
     // entries of U that idle passes would turn from -0.0 into +0.0.
-    PassCheck *chk = nullptr;
-    if (g_skip_idle_local && sc.local_passes > 0) TRY(pass_check(&chk));
+    GlideVcycleWork *chk = sc.local_skip && sc.local_passes > 0 ? work : nullptr;
     int *flags = reinterpret_cast<int *>(A.scratch);
     for (int pass = 0; pass < sc.local_passes; pass++) {
         TRY(launch_residual(A.g, p, A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma, st));
@@ -1168,13 +1191,13 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
             else if (pass >= 2)
                 TRY(read_flag(chk, k ^ 1, &any));   // pass - 1 (pass 0 was active)
             if (!any) {
-                g_local_passes_skipped += sc.local_passes - pass;
+                chk->skipped += sc.local_passes - pass;
                 break;
             }
+            chk->run++;
         } else {
             TRY(launch_activation(A.g, A.r, A.U, A.gamma, A.active, st));
         }
-        g_local_passes_run++;
         TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, A.active, sc.local_sweeps, sc.n_newton,
                           A.scratch, st));
     }
@@ -1259,19 +1282,52 @@ int adjoint_vc(const GlideLevel *L, int n, int k, const GlideSchedule &sc, hipSt
 // ===========================================================================
 extern "C" {
 
-const char *glide_last_error(void) { return g_last_error.c_str(); }
-int glide_set_local_skip(int on)
+size_t glide_last_error(char *buf, size_t cap)
 {
-    g_skip_idle_local = on != 0;
+    const size_t n = g_last_error.size();
+    if (buf && cap > 0) {
+        const size_t m = n < cap - 1 ? n : cap - 1;
+        std::memcpy(buf, g_last_error.data(), m);
+        buf[m] = '\0';
+    }
+    return n;
+}
+
+int glide_vcycle_work_create(GlideVcycleWork **out)
+{
+    if (!out) return fail(GLIDE_E_NULL, "vcycle_work_create: out is NULL");
+    *out = nullptr;
+    GlideVcycleWork *w = new GlideVcycleWork();
+    hipError_t e = hipGetDevice(&w->device);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&w->slot, 2 * sizeof(int), hipHostMallocDefault);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipEventCreateWithFlags(&w->ev[k], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        glide_vcycle_work_destroy(w);
+        return fail((int)e, std::string("vcycle_work_create: ") + hipGetErrorString(e));
+    }
+    w->slot[0] = w->slot[1] = 0;
+    *out = w;
     return GLIDE_OK;
 }
 
-int glide_local_pass_counts(long *run, long *skipped, int reset)
+int glide_vcycle_work_destroy(GlideVcycleWork *w)
 {
-    if (!run || !skipped) return fail(GLIDE_E_NULL, "local_pass_counts: NULL argument");
-    *run = g_local_passes_run;
-    *skipped = g_local_passes_skipped;
-    if (reset) g_local_passes_run = g_local_passes_skipped = 0;
+    if (!w) return GLIDE_OK;
+    for (int k = 0; k < 2; k++)
+        if (w->pending[k] && w->ev[k]) (void)hipEventSynchronize(w->ev[k]);
+    for (int k = 0; k < 2; k++)
+        if (w->ev[k]) (void)hipEventDestroy(w->ev[k]);
+    if (w->slot) (void)hipHostFree(w->slot);
+    delete w;
+    return GLIDE_OK;
+}
+
+int glide_vcycle_work_counts(GlideVcycleWork *w, long *run, long *skipped, int reset)
+{
+    if (!w || !run || !skipped) return fail(GLIDE_E_NULL, "vcycle_work_counts: NULL argument");
+    *run = w->run;
+    *skipped = w->skipped;
+    if (reset) w->run = w->skipped = 0;
     return GLIDE_OK;
 }
 
@@ -1381,6 +1437,7 @@ int glide_adjoint_vcycle(const GlideLevel *L, int n, const GlideSchedule *sched,
 {
     TRY(check_adjoint_levels(L, n));
     GlideSchedule sc = sched ? *sched : default_schedule();
+    TRY(check_schedule(sc));
     return adjoint_vc(L, n, 0, sc, S(stream));
 }
 
@@ -1405,14 +1462,16 @@ int glide_restrict_u(const double *uf, double *uc, int nyc, int nxc, void *strea
 {
     if (!uf || !uc) return fail(GLIDE_E_NULL, "restrict_u: NULL");
     if (nyc < 1 || nxc < 1) return fail(GLIDE_E_SIZE, "restrict_u: size");
-    hipLaunchKernelGGL(k_restrict_u, dim3(blocks1d(n_u(nyc, nxc))), dim3(256), 0, S(stream), uf, uc, nullptr, nyc, nxc);
+    hipLaunchKernelGGL(k_restrict_u, dim3(blocks1d(n_u(nyc, nxc))), dim3(256), 0, S(stream), uf, uc, nullptr, nyc, nxc,
+                       1.0);
     return check_launch("restrict_u");
 }
 int glide_restrict_v(const double *vf, double *vc, int nyc, int nxc, void *stream)
 {
     if (!vf || !vc) return fail(GLIDE_E_NULL, "restrict_v: NULL");
     if (nyc < 1 || nxc < 1) return fail(GLIDE_E_SIZE, "restrict_v: size");
-    hipLaunchKernelGGL(k_restrict_v, dim3(blocks1d(n_v(nyc, nxc))), dim3(256), 0, S(stream), vf, vc, nullptr, nyc, nxc);
+    hipLaunchKernelGGL(k_restrict_v, dim3(blocks1d(n_v(nyc, nxc))), dim3(256), 0, S(stream), vf, vc, nullptr, nyc, nxc,
+                       1.0);
     return check_launch("restrict_v");
 }
 int glide_restrict_cell(const double *hf, double *hc, int nyc, int nxc, void *stream)
@@ -1488,11 +1547,17 @@ int glide_propagate_geometry(const GlideLevel *L, int n, double thklim, void *st
     return GLIDE_OK;
 }
 
-int glide_fascd_vcycle(const GlideLevel *L, int n, double thklim, const GlideSchedule *sched, void *stream)
+int glide_fascd_vcycle(const GlideLevel *L, int n, double thklim, const GlideSchedule *sched, GlideVcycleWork *work,
+                       void *stream)
 {
     TRY(check_levels(L, n));
     GlideSchedule sc = sched ? *sched : default_schedule();
-    return fascd(L, n, 0, true, thklim, sc, S(stream));
+    TRY(check_schedule(sc));
+    if (sc.local_skip && sc.local_passes > 0) {
+        if (!work) return fail(GLIDE_E_NULL, "fascd_vcycle: local_skip needs a GlideVcycleWork");
+        TRY(work_begin(work, S(stream)));
+    }
+    return fascd(L, n, 0, true, thklim, sc, work, S(stream));
 }
 
 int glide_forward_setup(GlideLevel *L, int n, double dt, void *stream)
@@ -1529,11 +1594,14 @@ int glide_residual_norm(const GlideLevel *level, double *out_host, void *stream)
     if (!level || !out_host) return fail(GLIDE_E_NULL, "residual_norm: NULL");
     TRY(check_geom(&level->g));
     const GlideLevel &A = *level;
+    if (!A.U || !A.f || !A.r || !A.bed || !A.B || !A.beta || !A.mask || !A.gamma || !A.scratch)
+        return fail(GLIDE_E_NULL, "residual_norm: the level has a NULL buffer");
     hipStream_t st = S(stream);
+    // partial sums and the result go to the scratch buffer (5*nc doubles):
+    // 1024 blocks where it holds 1025 values, fewer on levels below 205 cells
+    const size_t cap = 5 * n_c(A.g.ny, A.g.nx) - 1;
+    const int nblk = (int)(cap < 1024 ? cap : 1024);
     TRY(launch_residual(A.g, to_params(A.p), A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma, st));
-    // partial sums go to the scratch buffer (5*nc doubles >= 1024 + 1)
-    const int nblk = 1024;
-    if (5 * n_c(A.g.ny, A.g.nx) < (size_t)nblk + 1) return fail(GLIDE_E_SIZE, "residual_norm: level too small");
     hipLaunchKernelGGL(k_norm_partial, dim3(nblk), dim3(256), 0, st, A.g.ny, A.g.nx, A.r, A.U, A.gamma, A.scratch);
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, st, nblk, A.scratch, A.scratch + nblk);
     TRY(check_launch("residual_norm"));

@@ -14,8 +14,6 @@ namespace glide {
 
 // records msg as glide_last_error() and returns code (host side)
 int set_error(int code, const char *msg);
-// whether V-cycles stop a level's local passes at the first idle pass
-bool skip_idle_local();
 
 constexpr int RB = 14;  // reference launch-block interior (glide/grid.py:184-186)
 constexpr int NXCD = 8; // XCDs per MI355X

@@ -67,7 +67,7 @@ class Ensemble:
         self.entries_per_phase = list(ent[: self.n_phases])
         self.cells_per_phase = list(cells[: self.n_phases])
         self.tiles: dict = {}
-        self.schedule = None
+        self.schedule = _lib.schedule()   # the reference's counts and restriction, idle-pass check on
         self.thklim = members[0].thklim
 
     def close(self):
@@ -86,9 +86,10 @@ class Ensemble:
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def set_schedule(self, pre=10, post=10, coarse=200, local_passes=10, local_sweeps=10, n_newton=10,
-                     adj_sweeps=10, adj_coarse=100):
-        self.schedule = _lib.GlideSchedule(pre, post, coarse, local_passes, local_sweeps, n_newton, adj_sweeps,
-                                           adj_coarse)
+                     adj_sweeps=10, adj_coarse=100, boundary_weight=1.0, local_skip=True):
+        """V-cycle settings (the same fields as IcePhysics.set_schedule)."""
+        self.schedule = _lib.schedule(pre, post, coarse, local_passes, local_sweeps, n_newton, adj_sweeps,
+                                      adj_coarse, boundary_weight, local_skip)
 
     def set_enabled(self, on):
         arr = (ctypes.c_ubyte * self.n)(*[1 if x else 0 for x in on])
@@ -112,8 +113,7 @@ class Ensemble:
         _lib.call("glide_batch_forward_setup", self._h, float(dt), self._stream())
 
     def vcycle(self):
-        sched = ctypes.byref(self.schedule) if self.schedule is not None else None
-        _lib.call("glide_batch_vcycle", self._h, float(self.thklim), sched, self._stream())
+        _lib.call("glide_batch_vcycle", self._h, float(self.thklim), ctypes.byref(self.schedule), self._stream())
 
     def residual_norms(self) -> np.ndarray:
         out = (ctypes.c_double * self.n)()

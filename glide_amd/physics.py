@@ -55,7 +55,8 @@ class IcePhysics:
         self.grids = [self.grid]
         for _ in range(n_levels - 1):
             self.grids.append(self.grids[-1].spawn_child())
-        self.schedule = None          # None = the reference's sweep counts
+        self.schedule = _lib.schedule()   # the reference's counts and restriction, idle-pass check on
+        self._work = None                 # GlideVcycleWork of the idle-pass check (created on first use)
         self._desc = None
         self.last_history: list[float] = []
         self.steps = 0                # forward steps taken (numbers the log records)
@@ -73,9 +74,19 @@ class IcePhysics:
         return ctypes.c_void_p(stream_handle(self.device))
 
     def set_schedule(self, pre=10, post=10, coarse=200, local_passes=10, local_sweeps=10, n_newton=10,
-                     adj_sweeps=10, adj_coarse=100):
-        self.schedule = _lib.GlideSchedule(pre, post, coarse, local_passes, local_sweeps, n_newton, adj_sweeps,
-                                           adj_coarse)
+                     adj_sweeps=10, adj_coarse=100, boundary_weight=1.0, local_skip=True):
+        """V-cycle settings.  The defaults are the reference's sweep counts
+        (glide/solver.py:71-220) and its residual restriction
+        (boundary_weight=1.0).  boundary_weight=0.5 halves the restricted
+        residual's boundary-face rows, whose one-sided stress term scales
+        like 1/dx where interior rows scale like 1/dx^2: with 1.0 the coarse
+        correction overshoots there and FASCD stalls on states with normal
+        stress at the domain edge; 0.5 converges (at the cost of departing
+        from the reference's iterates).  local_skip stops a level's local
+        passes at the first idle pass (same results up to the sign of
+        zeros; the V-cycle then waits on the GPU between passes)."""
+        self.schedule = _lib.schedule(pre, post, coarse, local_passes, local_sweeps, n_newton, adj_sweeps,
+                                      adj_coarse, boundary_weight, local_skip)
 
     # ------------------------------------------------------------ inputs
     def _check_finite(self, name: str, t: torch.Tensor):
@@ -132,8 +143,20 @@ class IcePhysics:
         return out.value
 
     def vcycle(self):
-        sched = ctypes.byref(self.schedule) if self.schedule is not None else None
-        _lib.call("glide_fascd_vcycle", self._levels(), self.n_levels, self.thklim, sched, self._stream())
+        work = None
+        if self.schedule.local_skip:
+            if self._work is None:
+                self._work = _lib.VcycleWork()
+            work = self._work.handle
+        _lib.call("glide_fascd_vcycle", self._levels(), self.n_levels, self.thklim, ctypes.byref(self.schedule), work,
+                  self._stream())
+
+    def local_pass_counts(self, reset: bool = False):
+        """(local passes run, local passes skipped) by this hierarchy's
+        V-cycles with the idle-pass check, since the last reset."""
+        if self._work is None:
+            return 0, 0
+        return self._work.counts(reset)
 
     def forward(self, dt, n_vcycles=3, verbose=False, update_geometry=True, tol=None, max_vcycles=None, log=None):
         """One implicit time step (glide/physics.py:141-209).
@@ -190,8 +213,7 @@ class IcePhysics:
 
     # ------------------------------------------------------------ adjoint
     def adjoint_vcycle(self):
-        sched = ctypes.byref(self.schedule) if self.schedule is not None else None
-        _lib.call("glide_adjoint_vcycle", self._levels(), self.n_levels, sched, self._stream())
+        _lib.call("glide_adjoint_vcycle", self._levels(), self.n_levels, ctypes.byref(self.schedule), self._stream())
 
     def grad_beta(self, reference_edges: bool = True) -> torch.Tensor:
         """lambda . dR/dbeta at the finest level (glide/grid.py:336-355).

@@ -10,18 +10,17 @@
  *
  * Conventions for every entry point:
  *   - plain device pointers and sizes, no framework types;
- *   - no device memory is allocated: scratch buffers are caller-provided
- *     (batch plans own theirs; the V-cycle's idle-pass check keeps 8 bytes of
- *     pinned host memory per host thread and device, see
- *     glide_set_local_skip);
+ *   - no memory is allocated except by the *_create calls: scratch buffers
+ *     are caller-provided (a batch plan and a V-cycle workspace own theirs);
  *   - work is enqueued on the caller's stream (hipStream_t passed as void*,
- *     NULL = the default stream); the entry points that read results back to
- *     the host (glide_residual_norm, glide_count_nonfinite,
- *     glide_batch_residual_norms, and the FASCD V-cycles while the idle-pass
- *     check is on) synchronise with it;
+ *     NULL = the default stream); only the entry points that read results
+ *     back to the host synchronise with it: glide_residual_norm,
+ *     glide_count_nonfinite, glide_batch_residual_norms, and the FASCD
+ *     V-cycles when their schedule asks for the idle-pass check
+ *     (GlideSchedule.local_skip);
  *   - return 0 on success, a negative GLIDE_E_* code for invalid arguments,
- *     or a positive hipError_t from the runtime; glide_last_error() gives a
- *     message for the last failure on the calling thread.
+ *     or a positive hipError_t from the runtime; glide_last_error() copies
+ *     the message of the last failure on the calling thread.
  */
 #ifndef GLIDE_HIP_H
 #define GLIDE_HIP_H
@@ -69,13 +68,38 @@ typedef struct GlideLevel {
     unsigned char *active;/* (ny/14+1)*(nx/14+1) bytes: local-smoother blocks */
 } GlideLevel;
 
-/* Sweep counts of one FASCD / adjoint V-cycle (hard-coded in the reference:
- * glide/solver.py:96,109,144,147-155,177,183,219; glide/grid.py:305). */
+/* One FASCD / adjoint V-cycle's settings.  The sweep counts are hard-coded
+ * in the reference (glide/solver.py:96,109,144,147-155,177,183,219;
+ * glide/grid.py:305); the values in the comments are the reference's and
+ * what a NULL schedule means.
+ *
+ * boundary_weight: factor on the boundary-face rows (u faces j = 0 and
+ *   j = nx, v faces i = 0 and i = ny) of the fine residual restricted to the
+ *   child (glide/solver.py:34-44,119-121 restricts them like interior rows:
+ *   1.0).  A boundary face borders one cell, so its momentum row is a
+ *   one-sided stress over dx (scaling as 1/h) where interior rows are stress
+ *   differences over dx (1/h^2); with 1.0 the coarse correction of those
+ *   rows is twice too strong and the V-cycle stalls wherever the solution
+ *   carries normal stress at the domain edge.  0.5 gives the interior rows'
+ *   scaling.  Values other than 1.0 depart from the reference's iterates.
+ * local_skip: nonzero stops a level's local passes at the first pass whose
+ *   activation marks no block (glide/solver.py:147-155 always runs all of
+ *   them; an idle pass changes nothing, and neither can any pass after it).
+ *   The host reads each pass's 4-byte any-block flag: it waits for pass 0's
+ *   before enqueuing its sweeps and for pass k-1's (already computed) when
+ *   enqueuing pass k, so at most one idle pass per level is still run, and
+ *   the V-cycle call returns only after the GPU has reached its last local
+ *   pass.  The only difference from running every pass is the sign of zero
+ *   entries of U that idle passes would turn from -0.0 into +0.0.  The
+ *   single-domain V-cycle needs a GlideVcycleWork for it; batch plans own
+ *   their flags. */
 typedef struct GlideSchedule {
     int pre, post, coarse;          /* 10, 10, 200 */
     int local_passes, local_sweeps; /* 10, 10 */
     int n_newton;                   /* 10 */
     int adj_sweeps, adj_coarse;     /* 10, 100 */
+    double boundary_weight;         /* 1.0 */
+    int local_skip;                 /* 0 */
 } GlideSchedule;
 
 /* ---- level operators ------------------------------------------------- */
@@ -172,18 +196,35 @@ int glide_restrict_parameters(const GlideLevel *levels, int n_levels, void *stre
 int glide_propagate_geometry(const GlideLevel *levels, int n_levels, double thklim,
                              void *stream);
 
+/* Host-side state of the single-domain V-cycle's idle-pass check: two
+ * pinned 4-byte flag slots, two events and the pass counters.  Created for
+ * the device current at creation (the only allocating V-cycle call; no
+ * reference counterpart).  A workspace serves one V-cycle at a time: calls
+ * that share one must not overlap (one host thread, or the caller's lock);
+ * moving it to another stream is safe (the first call there waits for flag
+ * copies still in flight on the previous one). */
+typedef struct GlideVcycleWork GlideVcycleWork;
+int glide_vcycle_work_create(GlideVcycleWork **out);
+int glide_vcycle_work_destroy(GlideVcycleWork *work);
+
+/* Local passes run and skipped by V-cycles that used this workspace since
+ * the last reset (reset != 0 zeroes them after reading). */
+int glide_vcycle_work_counts(GlideVcycleWork *work, long *run, long *skipped, int reset);
+
 /* One FASCD V-cycle from the finest level (glide/solver.py:71-157,
  * fascd_vcycle(grid, thklim, finest=True)).  sched == NULL uses the
- * reference's counts. */
+ * reference's settings.  work may be NULL unless sched->local_skip is set;
+ * without the idle-pass check the call never synchronises or allocates. */
 int glide_fascd_vcycle(const GlideLevel *levels, int n_levels, double thklim,
-                       const GlideSchedule *sched, void *stream);
+                       const GlideSchedule *sched, GlideVcycleWork *work, void *stream);
 
 /* f_H = H_prev / dt + smb on the finest level and dt on every level
  * (glide/physics.py:166-173). */
 int glide_forward_setup(GlideLevel *levels, int n_levels, double dt, void *stream);
 
 /* ||(r_u, r_v, FB(r_H, H - gamma))||_2 of the finest level into *out_host
- * (the verbose norm of glide/physics.py:176-203); synchronises the stream. */
+ * (the verbose norm of glide/physics.py:176-203); writes the residual into
+ * level->r and partial sums into level->scratch, synchronises the stream. */
 int glide_residual_norm(const GlideLevel *level, double *out_host, void *stream);
 
 /* Number of NaN / +-Inf entries among x[0..n) into *count_host, using the
@@ -237,7 +278,9 @@ int glide_batch_info(const GlideBatch *batch, int *n_phases, int *entries_per_ph
  * finest level: glide_forward_setup for all members (glide/physics.py:166-173). */
 int glide_batch_forward_setup(GlideBatch *batch, double dt, void *stream);
 
-/* One FASCD V-cycle of every enabled member (glide/solver.py:71-157). */
+/* One FASCD V-cycle of every enabled member (glide/solver.py:71-157).
+ * With sched->local_skip a pass is idle when no member's activation marks a
+ * block; the flags are the plan's. */
 int glide_batch_vcycle(GlideBatch *batch, double thklim, const GlideSchedule *sched,
                        void *stream);
 
@@ -265,36 +308,22 @@ int glide_batch_time_kernel(GlideBatch *batch, int kernel, int phase, int ty, in
 /* Workgroups per CU of the persistent single-level launches: kernel 0 =
  * the Vanka sweep (default 64, i.e. one tile per workgroup), 1 = the
  * residual (default 4).  Each launch has
- * min(tiles, CUs x wg_per_cu) workgroups looping over its tiles; results do
- * not depend on the setting.  Process-wide; no reference counterpart. */
+ * min(tiles, CUs x wg_per_cu) workgroups looping over its tiles.  A launch
+ * shape only: results do not depend on it.  Process-wide (atomic); no
+ * reference counterpart. */
 int glide_set_persistence(int kernel, int wg_per_cu);
 
 /* Levels with at most max_cells cells run their Vanka sweeps row-split (four
- * waves per 64 cells, each assembling part of the local block; same results,
- * shorter per-wave instruction stream for latency-bound small levels); 0
- * disables it.  Process-wide; no reference counterpart. */
+ * waves per 64 cells, each assembling part of the local block; shorter
+ * per-wave instruction stream for latency-bound small levels); 0 disables
+ * it.  A launch shape only: results are bitwise the same.  Process-wide
+ * (atomic); no reference counterpart. */
 int glide_set_row_split(long max_cells);
 
-/* on != 0: the FASCD V-cycle stops a level's local passes once a pass's
- * activation marks no block (glide/solver.py:147-155 runs all ten; an idle
- * pass changes nothing, and neither can any pass after it).  The host reads
- * each pass's 4-byte any-block flag: it waits for pass 0's before enqueuing
- * its sweeps, and for pass k-1's (already computed) when enqueuing pass k,
- * so at most one idle pass per level is still run.  The V-cycle call then
- * returns only after the GPU has reached its last local pass.  First use on
- * a host thread creates 8 bytes of pinned host memory and two events, kept
- * for the life of the process.  The batched V-cycle does the same over all
- * enabled members (a pass is idle when no member's activation marks a
- * block), with its flags owned by the batch plan.  Process-wide switch;
- * default on. */
-int glide_set_local_skip(int on);
-
-/* Local passes run and skipped by the single-domain FASCD V-cycle since the
- * last reset (process-wide counters; reset != 0 zeroes them after reading). */
-int glide_local_pass_counts(long *run, long *skipped, int reset);
-
-/* Message of the last failure on this thread ("" if none). */
-const char *glide_last_error(void);
+/* Copies the message of the last failure on this thread ("" if none) into
+ * buf (at most cap - 1 characters and a terminating NUL; buf may be NULL
+ * when cap is 0) and returns the message's full length. */
+size_t glide_last_error(char *buf, size_t cap);
 
 /* Library version string. */
 const char *glide_version(void);

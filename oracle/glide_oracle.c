@@ -1478,6 +1478,7 @@ typedef struct {
     OLevel L[OMAXLEV];
     int sweeps_pre, sweeps_post, sweeps_coarse, local_passes, local_sweeps, n_newton;
     int adj_sweeps, adj_sweeps_coarse;
+    double bweight;   /* weight of the boundary-face rows of restricted residuals (1 = solver.py) */
 } OHier;
 
 static double *zalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
@@ -1522,6 +1523,7 @@ OHier *oh_create(int ny, int nx, double dx, int nlev, const OParams *p, double t
     h->sweeps_pre = 10; h->sweeps_post = 10; h->sweeps_coarse = 200;
     h->local_passes = 10; h->local_sweeps = 10; h->n_newton = 10;
     h->adj_sweeps = 10; h->adj_sweeps_coarse = 100;
+    h->bweight = 1.0;
     OParams pc = *p;
     pc.gl_sigmoid_c = 0.1; pc.gl_derivatives = 0;
     for (int k = 0; k < nlev; k++)
@@ -1542,6 +1544,30 @@ void oh_set_schedule(OHier *h, int pre, int post, int coarse, int local_passes, 
     h->sweeps_pre = pre; h->sweeps_post = post; h->sweeps_coarse = coarse;
     h->local_passes = local_passes; h->local_sweeps = local_sweeps; h->n_newton = n_newton;
     h->adj_sweeps = adj_sweeps; h->adj_sweeps_coarse = adj_coarse;
+}
+
+/* Weight of the boundary-face rows (u faces j = 0 and j = nx, v faces
+ * i = 0 and i = ny) of every restricted residual.  solver.py:34-44 restricts
+ * them like interior rows (weight 1).  A boundary face has one neighbouring
+ * cell, so its momentum row is a one-sided stress sigma / dx (scaling as
+ * 1/h) where interior rows are stress differences (1/h^2); restricted with
+ * weight 1 the coarse correction of those rows is twice too strong.  0.5
+ * restores the interior rows' scaling. */
+void oh_set_boundary_weight(OHier *h, double w) { h->bweight = w; }
+
+/* dst (a coarse state-sized vector) *= w on its boundary-face rows */
+static void weight_boundary_faces(OLevel *C, double *dst, double w)
+{
+    int ny = C->g.ny, nx = C->g.nx;
+    double *u = dst, *v = dst + C->nu;
+    for (int i = 0; i < ny; i++) {
+        u[(size_t)i * (nx + 1)] *= w;
+        u[(size_t)i * (nx + 1) + nx] *= w;
+    }
+    for (int j = 0; j < nx; j++) {
+        v[j] *= w;
+        v[(size_t)ny * nx + j] *= w;
+    }
 }
 
 /* field access for the ctypes wrapper */
@@ -1649,6 +1675,7 @@ static void fascd(OHier *h, int k, int finest)
     memcpy(C->w, C->U, C->nt * sizeof(double));
     lev_residual(L, L->r, L->f);
     restrict_vec(L, C, L->r, C->r);
+    if (h->bweight != 1.0) weight_boundary_faces(C, C->r, h->bweight);
     lev_residual(C, C->F, C->Z);
     for (size_t t = 0; t < C->nt; t++) C->f[t] = C->F[t] - C->r[t];
     fascd(h, k + 1, 0);

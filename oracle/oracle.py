@@ -70,6 +70,7 @@ def _lib(libm: bool = False):
         "oh_create": (c_void_p, [c_int, c_int, c_double, c_int, P, c_double]),
         "oh_destroy": (None, [c_void_p]),
         "oh_set_schedule": (None, [c_void_p] + [c_int] * 8),
+        "oh_set_boundary_weight": (None, [c_void_p, c_double]),
         "oh_field": (DP, [c_void_p, c_int, c_int]),
         "oh_set_geometry": (None, [c_void_p, DP, DP]),
         "oh_set_parameters": (None, [c_void_p, DP, DP, DP]),
@@ -338,9 +339,10 @@ class Hierarchy:
             self.h = None
 
     def set_schedule(self, pre=10, post=10, coarse=200, local_passes=10, local_sweeps=10, n_newton=10,
-                     adj_sweeps=10, adj_coarse=100):
+                     adj_sweeps=10, adj_coarse=100, boundary_weight=1.0):
         self.lib.oh_set_schedule(self.h, pre, post, coarse, local_passes, local_sweeps, n_newton,
                                  adj_sweeps, adj_coarse)
+        self.lib.oh_set_boundary_weight(self.h, float(boundary_weight))
 
     def field(self, name: str, level: int = 0) -> np.ndarray:
         ny, nx = self.ny >> level, self.nx >> level

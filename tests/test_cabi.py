@@ -63,6 +63,66 @@ def test_errors_are_reported_without_a_gpu():
     assert _lib.load().glide_version().startswith(b"glide_amd")
 
 
+def test_last_error_copies_and_truncates():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    with pytest.raises(_lib.GlideError):
+        _lib.call("glide_residual", None, None, None, None, None, None, None, None, None, None, None, None)
+    f = _lib.fn("glide_last_error")
+    n = f(None, 0)
+    full = _lib.last_error()
+    assert n == len(full) > 0 and "NULL" in full
+    buf = ctypes.create_string_buffer(8)
+    assert f(buf, 8) == n
+    assert buf.value == full.encode()[:7]
+
+
+def fake_levels(sizes):
+    """Level descriptors with non-NULL dummy pointers (never dereferenced:
+    the calls below fail validation before any device work)."""
+    lv = (_lib.GlideLevel * len(sizes))()
+    for k, n in enumerate(sizes):
+        lv[k].g = _lib.GlideGeom(n, n, 1000.0 * (1 << k), 1.0)
+        lv[k].p = _lib.GlideParams(3.0, 1e-5, 1e-3, 1.0, 0.1, 0)
+        for name in _lib._LEVEL_PTRS:
+            setattr(lv[k], name, 8)
+    return lv
+
+
+def test_vcycle_schedule_validation_without_a_gpu():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lv = fake_levels((16, 8))
+    sc = _lib.schedule(local_skip=True)
+    with pytest.raises(_lib.GlideError) as e:     # the idle-pass check needs a workspace
+        _lib.call("glide_fascd_vcycle", lv, 2, 0.1, ctypes.byref(sc), None, None)
+    assert e.value.code == _lib.GLIDE_E_NULL and "GlideVcycleWork" in str(e.value)
+    sc = _lib.schedule(pre=-1, local_skip=False)
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_fascd_vcycle", lv, 2, 0.1, ctypes.byref(sc), None, None)
+    assert e.value.code == _lib.GLIDE_E_ARG
+    sc = _lib.schedule(boundary_weight=float("nan"), local_skip=False)
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_fascd_vcycle", lv, 2, 0.1, ctypes.byref(sc), None, None)
+    assert e.value.code == _lib.GLIDE_E_ARG
+    with pytest.raises(_lib.GlideError) as e:
+        _lib.call("glide_vcycle_work_counts", None, None, None, 0)
+    assert e.value.code == _lib.GLIDE_E_NULL
+
+
+def test_residual_norm_rejects_null_buffers_before_launching():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lv = fake_levels((16,))
+    out = ctypes.c_double()
+    for name in ("r", "U", "f", "gamma", "scratch"):
+        lv[0].__setattr__(name, None)
+        with pytest.raises(_lib.GlideError) as e:
+            _lib.call("glide_residual_norm", ctypes.byref(lv[0]), ctypes.byref(out), None)
+        assert e.value.code == _lib.GLIDE_E_NULL
+        lv[0].__setattr__(name, 8)
+
+
 def test_batch_errors_are_reported_without_a_gpu():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("library not built")

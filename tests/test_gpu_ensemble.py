@@ -58,6 +58,27 @@ def test_batched_vcycles_bitwise_equal_individual(members):
     ens.close()
 
 
+def test_batched_boundary_weight_bitwise_equal_individual(members):
+    sched = dict(SCHED, boundary_weight=0.5)
+    solo = [make(d, sched=False) for d in members]
+    for ph in solo:
+        ph.set_schedule(**sched)
+        ph.forward(10.0, n_vcycles=2)
+    batch = [make(d, sched=False) for d in members]
+    ens = Ensemble(batch)
+    ens.set_schedule(**sched)
+    ens.forward(10.0, n_vcycles=2)
+    torch.cuda.synchronize()
+    ref = [make(d) for d in members]          # the reference restriction gives other states
+    for ph in ref:
+        ph.forward(10.0, n_vcycles=2)
+    torch.cuda.synchronize()
+    for k, (a, b) in enumerate(zip(solo, batch)):
+        assert np.array_equal(state(a), state(b)), f"member {k}"
+    assert any(not np.array_equal(state(a), state(c)) for a, c in zip(solo, ref))
+    ens.close()
+
+
 def test_tile_shapes_do_not_change_results(members):
     ref = [make(d) for d in members]
     e1 = Ensemble(ref)

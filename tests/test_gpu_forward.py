@@ -208,3 +208,72 @@ def test_forward_dome_128_five_steps():
         errs = [rel(a.cpu().numpy(), b) for a, b in ((u, u0), (v, v0), (H, H0))]
         print(f"step {step}: rel L2 u {errs[0]:.2e} v {errs[1]:.2e} H {errs[2]:.2e}", flush=True)
         assert max(errs) <= 1e-10, errs
+
+
+def test_boundary_weight_vcycles_bitwise():
+    """Halved boundary rows of the restricted residual: GPU V-cycles equal
+    the oracle's bit for bit."""
+    w = S.ice_sheet(64, 64, 1500.0, seed=20000)
+    h, ph = _pair(w, 3)
+    sched = dict(pre=4, post=4, coarse=20, local_passes=2, local_sweeps=2, boundary_weight=0.5)
+    h.set_schedule(**sched)
+    ph.set_schedule(**sched)
+    u0, v0, H0 = h.forward(1.0, n_vcycles=2)
+    u, v, H = ph.forward(1.0, n_vcycles=2)
+    torch.cuda.synchronize()
+    for a, b in ((u, u0), (v, v0), (H, H0)):
+        assert np.array_equal(a.cpu().numpy(), b), np.abs(a.cpu().numpy() - b).max()
+
+
+def test_boundary_weight_converges_where_the_reference_stalls():
+    """128^2 ice sheet, 4 levels, dt = 1 a, full schedule: to 1e-8 with
+    boundary_weight 0.5 (each V-cycle's state equal to the oracle's), while
+    the reference restriction stays above 1e-6."""
+    w = S.ice_sheet(128, 128, 1500.0, seed=20000)
+    h, ph = _pair(w, 4)
+    h.set_schedule(boundary_weight=0.5)
+    ph.set_schedule(boundary_weight=0.5)
+    (u0, v0, H0), hist0 = h.forward(1.0, n_vcycles=20, tol=1e-8, history=True)
+    u, v, H = ph.forward(1.0, tol=1e-8, max_vcycles=20)
+    torch.cuda.synchronize()
+    print("weighted:", len(ph.last_history), ph.last_history[-1], flush=True)
+    assert len(ph.last_history) == len(hist0) <= 14 and ph.last_history[-1] <= 1e-8
+    np.testing.assert_allclose(ph.last_history, hist0, rtol=1e-9)
+    for a, b in ((u, u0), (v, v0), (H, H0)):
+        assert np.array_equal(a.cpu().numpy(), b)
+    _, ref = _pair(w, 4)
+    ref.forward(1.0, tol=1e-8, max_vcycles=20)
+    assert len(ref.last_history) == 20 and min(ref.last_history) > 1e-6
+
+
+@pytest.mark.parametrize("shape", [(8, 10), (64, 64), (130, 70)])
+def test_residual_norm_matches_oracle(shape):
+    """glide_residual_norm against the oracle's norm (same residual, FP64
+    sums in another order) on random states with cells at the thickness
+    limit, so that the FB branch matters (glide/physics.py:176-203)."""
+    ny, nx = shape
+    s = random_state(ny, nx, seed=ny + nx)
+    H = s["H"].copy()
+    H.ravel()[::7] = 0.1
+    s["U"][-ny * nx:] = H.ravel()
+    ph = IcePhysics(ny, nx, s["dx"], n_levels=1)
+    ph.set_geometry(s["bed"], H)
+    ph.set_parameters(B=s["B"], beta=s["beta"], smb=0.0)
+    for g in ph.grids:
+        g.dt = s["dt"]
+    ph.grid.U.copy_(T(s["U"]))
+    ph.grid.f.copy_(T(s["f"]))
+    h = O.Hierarchy(ny, nx, s["dx"], n_levels=1)
+    h.set_geometry(s["bed"], H)
+    h.set_parameters(B=s["B"], beta=s["beta"], smb=0.0)
+    h.forward(s["dt"], n_vcycles=0, update_geometry=False)
+    h.field("U")[:] = s["U"]
+    h.field("f")[:] = s["f"]
+    ref = h.residual_norm()
+    got = ph.residual_norm()
+    print(shape, got, ref, flush=True)
+    assert abs(got - ref) <= 1e-13 * ref
+    r = ph.grid.r.cpu().numpy()
+    r0 = O.residual(ny, nx, s["dx"], s["dt"], O.params(), s["U"], s["f"], s["bed"], s["B"], s["beta"],
+                    np.zeros((ny, nx)), np.full((ny, nx), 0.1))
+    assert np.array_equal(r, r0)

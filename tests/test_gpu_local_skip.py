@@ -1,10 +1,9 @@
-"""Stopping a level's local passes at the first idle pass (glide_set_local_skip)
-changes nothing but the work done: a forward solve of a seeded ice sheet,
-alone and as a batched ensemble, ends in the same state with the switch on
-and off, and the pass counters show that passes were skipped.  (Idle passes
-can only flip -0.0 entries to +0.0, which compare equal.)"""
-import ctypes
-
+"""Stopping a level's local passes at the first idle pass (the schedule's
+local_skip) changes nothing but the work done: a forward solve of a seeded
+ice sheet, alone and as a batched ensemble, ends in the same state with the
+check on and off, and the workspace's pass counters show that passes were
+skipped.  (Idle passes can only flip -0.0 entries to +0.0, which compare
+equal.)"""
 import pytest
 
 torch = pytest.importorskip("torch")
@@ -13,51 +12,57 @@ pytestmark = pytest.mark.gpu
 if not torch.cuda.is_available():  # pragma: no cover
     pytest.skip("no GPU", allow_module_level=True)
 
-from glide_amd import _lib  # noqa: E402
 from glide_amd.ensemble import Ensemble  # noqa: E402
 from glide_amd.train_warmstart import TEST_SEED0, member  # noqa: E402
 
 
-def counts():
-    run, skipped = ctypes.c_long(), ctypes.c_long()
-    _lib.call("glide_local_pass_counts", ctypes.byref(run), ctypes.byref(skipped), 1)
-    return run.value, skipped.value
-
-
-@pytest.fixture(autouse=True)
-def _restore():
-    yield
-    _lib.call("glide_set_local_skip", 1)
-
-
-def solve(skip, n=256, levels=4, cycles=6):
-    _lib.call("glide_set_local_skip", int(skip))
+def solve(skip, n=256, levels=4, cycles=6, boundary_weight=1.0):
     ph = member(TEST_SEED0 + 3, n, dx=1500.0, n_levels=levels)
-    counts()
+    ph.set_schedule(local_skip=skip, boundary_weight=boundary_weight)
     ph.forward(1.0, n_vcycles=cycles, update_geometry=False)
     torch.cuda.synchronize()
-    return (ph.grid.u.clone(), ph.grid.v.clone(), ph.grid.H.clone()), counts()
+    return (ph.grid.u.clone(), ph.grid.v.clone(), ph.grid.H.clone()), ph.local_pass_counts(reset=True)
 
 
-def test_single_domain_same_state_fewer_passes():
-    off, (run_off, skip_off) = solve(False)
-    on, (run_on, skip_on) = solve(True)
-    print(f"local passes: without skipping {run_off} run; with skipping {run_on} run, {skip_on} skipped", flush=True)
-    assert skip_off == 0 and run_off == 6 * 3 * 10
-    assert run_on + skip_on == run_off and skip_on > 0
+@pytest.mark.parametrize("bw", [1.0, 0.5])
+def test_single_domain_same_state_fewer_passes(bw):
+    off, (run_off, skip_off) = solve(False, boundary_weight=bw)
+    on, (run_on, skip_on) = solve(True, boundary_weight=bw)
+    print(f"bw {bw}: with skipping {run_on} local passes run, {skip_on} skipped", flush=True)
+    assert (run_off, skip_off) == (0, 0)          # no workspace without the check
+    assert run_on + skip_on == 6 * 3 * 10 and skip_on > 0
     assert all(torch.equal(a, b) for a, b in zip(off, on))
 
 
 def test_batched_same_state():
     states = {}
     for skip in (False, True):
-        _lib.call("glide_set_local_skip", int(skip))
         mem = [member(TEST_SEED0 + 3 + k, n, dx=1500.0, n_levels=lv)
                for k, (n, lv) in enumerate([(256, 4), (128, 3), (256, 4)])]
         ens = Ensemble(mem)
+        ens.set_schedule(local_skip=skip)
         ens.forward(1.0, n_vcycles=4, update_geometry=False)
         torch.cuda.synchronize()
         states[skip] = [(m.grid.u.clone(), m.grid.v.clone(), m.grid.H.clone()) for m in mem]
         ens.close()
     for a, b in zip(states[False], states[True]):
         assert all(torch.equal(x, y) for x, y in zip(a, b))
+
+
+def test_workspace_moves_between_streams():
+    """One workspace used by V-cycles on two streams in turn gives the state
+    of V-cycles on the default stream alone."""
+    ref, _ = solve(True, cycles=3)
+    ph = member(TEST_SEED0 + 3, 256, dx=1500.0, n_levels=4)
+    ph.set_schedule(local_skip=True)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    ph.forward(1.0, n_vcycles=0, update_geometry=False)
+    torch.cuda.synchronize()
+    for k in range(3):
+        s = s1 if k % 2 == 0 else s2
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            ph.vcycle()
+        torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    assert all(torch.equal(a, b) for a, b in zip(ref, (ph.grid.u, ph.grid.v, ph.grid.H)))

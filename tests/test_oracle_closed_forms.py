@@ -1,0 +1,239 @@
+"""Closed-form pins of the oracle (CPU): the residual on manufactured states,
+the relative-residual norm (with its Fischer-Burmeister branch), the local
+smoother's error mask, the FAS fixed point of the V-cycle, and the boundary
+weighting of the restricted residual.
+
+Expected values come from the continuum definitions, not from the oracle's
+code: for a uniform strain rate the viscosity is a closed form, the stress
+divergence vanishes in the interior and only the one-sided boundary rows,
+the drag and the mass flux remain (glide/cuda/residuals.cu:6-292,
+viscosity.cu:61-112, stress.cu, flux.cu; glide/physics.py:176-203;
+glide/solver.py:71-157).
+"""
+import numpy as np
+import pytest
+
+from glide_amd import synthetic as S
+from oracle import oracle as O
+
+NY, NX, DX, DT = 8, 10, 1000.0, 10.0
+H0, BED0, BETA0, B0 = 500.0, 100.0, 0.2, 23.9494
+PRM = O.params()                         # n = 3, eps_reg = 1e-5, water_drag = 1e-3, calving 1, c = 0.1
+
+
+def grounded():
+    """g = sigmoid(c (bed + 0.917 H)) with c z clamped to 20 (common.cu:67-76)."""
+    z = min(max(0.1 * (BED0 + 0.917 * H0), -20.0), 20.0)
+    return 1.0 / (1.0 + np.exp(-z))
+
+
+def eta_closed(e2):
+    """0.5 B (eps_II^2 + eps_reg)^((1 - n) / 2n) for n = 3."""
+    return 0.5 * B0 * (e2 + 1e-5) ** (-1.0 / 3.0)
+
+
+def residual(u, v, H=None):
+    H = np.full((NY, NX), H0) if H is None else H
+    U = O.join(u, v, H)
+    f = np.zeros_like(U)
+    c = lambda x: np.full((NY, NX), x)  # noqa: E731
+    r = O.residual(NY, NX, DX, DT, PRM, U, f, c(BED0), c(B0), c(BETA0), c(0.0), c(0.1))
+    return O.split(r, NY, NX)
+
+
+def drag():
+    g = grounded()
+    return g * BETA0 + (1.0 - g) * 1e-3, g
+
+
+def close(a, b, scale):
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * scale)
+
+
+def test_residual_uniform_extension_in_x():
+    """u = a x on faces (u_j = a j dx), v = 0, uniform H, flat grounded bed."""
+    a = 2e-3
+    u = np.tile(a * DX * np.arange(NX + 1.0), (NY, 1))
+    v = np.zeros((NY + 1, NX))
+    ru, rv, rH = residual(u, v)
+    eta = eta_closed(a * a)
+    be, g = drag()
+    sxx, syy = 2 * eta * H0 * 2 * a, 2 * eta * H0 * a      # 2 eta H (2 u_x + v_y), 2 eta H (u_x + 2 v_y)
+    exp_u = -be * u                                          # interior: stress divergence 0, drag only
+    exp_u[:, 0] += sxx / DX                                  # left edge: sigma_xx of cell 0 only
+    exp_u[:, NX] -= sxx / DX                                 # right edge: - sigma_xx of cell nx-1 only
+    exp_v = np.zeros((NY + 1, NX))
+    exp_v[0] = -syy / DX                                     # top edge (i = 0): - sigma_yy of cell 0
+    exp_v[NY] = syy / DX                                     # bottom edge: + sigma_yy of cell ny-1
+    q = H0 * u.copy()                                        # upwind flux of uniform H is H u; 0 on edge faces
+    q[:, 0] = q[:, NX] = 0.0
+    exp_H = H0 / DT + (1.0 - g) * H0 + (q[:, 1:] - q[:, :-1]) / DX
+    close(ru, exp_u, sxx / DX)
+    close(rv, exp_v, syy / DX)
+    close(rH, exp_H, H0 / DT)
+
+
+def test_residual_uniform_extension_in_y():
+    """v = b y (v_i = b (ny - i) dx, so v_top - v_bottom = b dx), u = 0."""
+    b = 1.5e-3
+    u = np.zeros((NY, NX + 1))
+    v = np.tile((b * DX * (NY - np.arange(NY + 1.0)))[:, None], (1, NX))
+    ru, rv, rH = residual(u, v)
+    eta = eta_closed(b * b)
+    be, g = drag()
+    sxx, syy = 2 * eta * H0 * b, 2 * eta * H0 * 2 * b
+    exp_u = np.zeros((NY, NX + 1))
+    exp_u[:, 0] = sxx / DX
+    exp_u[:, NX] = -sxx / DX
+    exp_v = -be * v
+    exp_v[0] -= syy / DX
+    exp_v[NY] += syy / DX
+    q = H0 * v.copy()
+    q[0] = q[NY] = 0.0
+    exp_H = H0 / DT + (1.0 - g) * H0 + (q[:-1] - q[1:]) / DX   # top flux in, bottom flux out
+    close(ru, exp_u, sxx / DX)
+    close(rv, exp_v, syy / DX)
+    close(rH, exp_H, H0 / DT)
+
+
+def test_residual_uniform_shear():
+    """u = -s i dx (u_top - u_bottom = s dx at every vertex), v = 0: the
+    shear strain rate is s/2 on every vertex, but a cell's viscosity only
+    counts its corners away from the domain edge, and sigma_xy lives on
+    interior vertices only."""
+    s = 1e-3
+    u = np.tile((-s * DX * np.arange(NY, dtype=float))[:, None], (1, NX + 1))
+    v = np.zeros((NY + 1, NX))
+    ru, rv, rH = residual(u, v)
+    i, j = np.meshgrid(np.arange(NY), np.arange(NX), indexing="ij")
+    corners = ((i > 0) & (j > 0)).astype(int) + ((i > 0) & (j < NX - 1)) + ((i < NY - 1) & (j > 0)) + \
+        ((i < NY - 1) & (j < NX - 1))
+    eta = eta_closed(0.25 * corners * (0.5 * s) ** 2)
+    sxy = np.zeros((NY + 1, NX + 1))                          # vertices (i, j), i in 0..ny, j in 0..nx
+    eH = 0.25 * H0 * (eta[:-1, :-1] + eta[:-1, 1:] + eta[1:, :-1] + eta[1:, 1:])
+    sxy[1:NY, 1:NX] = 2 * eH * (0.5 * s)
+    be, g = drag()
+    exp_u = (sxy[:-1, :] - sxy[1:, :]) / DX - be * u          # face (i, j): vertex (i, j) minus vertex (i+1, j)
+    exp_v = (sxy[:, 1:] - sxy[:, :-1]) / DX                   # face (i, j): vertex (i, j+1) minus vertex (i, j)
+    q = H0 * u.copy()
+    q[:, 0] = q[:, NX] = 0.0
+    exp_H = H0 / DT + (1.0 - g) * H0 + (q[:, 1:] - q[:, :-1]) / DX
+    scale = sxy.max() / DX
+    close(ru, exp_u, scale)
+    close(rv, exp_v, scale)
+    close(rH, exp_H, H0 / DT)
+    assert scale > 0 and len(np.unique(np.round(eta, 12))) == 3   # interior, edge and corner cells differ
+
+
+def hierarchy_with_residual(target_H, H, n_levels=2):
+    """A hierarchy at a slab state whose forcing is chosen so that the
+    finest r_H equals target_H (f_H = F_H(U) - target) and r_u = r_v = 0
+    up to rounding."""
+    ny, nx = H.shape
+    h = O.Hierarchy(ny, nx, DX, n_levels=n_levels)
+    bed = np.full((ny, nx), BED0)
+    h.set_geometry(bed, H)
+    h.set_parameters(B=B0, beta=BETA0, smb=0.0)
+    h.forward(DT, n_vcycles=0, update_geometry=False)      # dt on every level
+    U = h.field("U")
+    rng = np.random.default_rng(5)
+    U[: U.size - ny * nx] = rng.normal(0.0, 50.0, U.size - ny * nx)
+    F = O.residual(ny, nx, DX, DT, PRM, U, np.zeros(U.size), bed, h.field("B"), h.field("beta"),
+                   np.zeros((ny, nx)), np.full((ny, nx), 0.1))
+    f = h.field("f")
+    f[:] = F
+    f[-ny * nx:] -= target_H.ravel()
+    return h
+
+
+def test_residual_norm_closed_form():
+    """||(r_u, r_v, FB(r_H, H - gamma))||: with r_u = r_v = 0, cells at the
+    thickness limit give FB(a, 0) = a - |a| (0 for a > 0, 2a for a < 0) and
+    cells with (a, b) = (3k, 4k) give FB = 3k + 4k - 5k = 2k."""
+    ny, nx = 8, 8
+    H = np.full((ny, nx), 0.1)
+    target = np.zeros((ny, nx))
+    target[0, :4] = [0.7, -0.25, 1.5, -2.0]                  # at the limit: contribute 0, -0.5, 0, -4
+    k = np.array([1.0, 2.5, 0.3, 40.0])
+    target[3, 2:6] = 3 * k
+    H[3, 2:6] = 0.1 + 4 * k                                  # b = H - gamma = 4k: FB = 2k
+    H[5:, :] = 900.0
+    target[6, 1], target[7, 7] = 1e-6, -3e-6                 # b = 899.9: FB = a - a^2/2b + ... = a to 1e-15
+    h = hierarchy_with_residual(target, H)
+    expected = np.sqrt(0.25 + 16.0 + np.sum((2 * k) ** 2) + 1e-12 + 9e-12)
+    r = O.split(O.residual(ny, nx, DX, DT, PRM, h.field("U"), h.field("f"), h.field("bed"), h.field("B"),
+                           h.field("beta"), h.field("mask"), h.field("gamma")), ny, nx)
+    assert max(np.abs(r[0]).max(), np.abs(r[1]).max()) < 1e-9          # momentum rows cancelled
+    assert h.residual_norm() == pytest.approx(expected, rel=1e-9)
+
+
+def test_error_mask_closed_form():
+    """error_mask = (|r_u right| + |r_u left| + |r_v bottom| + |r_v top| +
+    |FB(r_H, H - gamma)|) > 0.01 (glide/solver.py:147-154)."""
+    ny, nx = 6, 7
+    ru = np.zeros((ny, nx + 1))
+    rv = np.zeros((ny + 1, nx))
+    rH = np.zeros((ny, nx))
+    H = np.full((ny, nx), 500.0)
+    gamma = np.full((ny, nx), 0.1)
+    ru[1, 3] = 0.011                  # one face above 0.01: both cells (1, 2) and (1, 3)
+    ru[4, 1] = -0.004                 # cell (4, 1): 0.004 + 0.004 + 0.004 = 0.012 from three faces
+    ru[4, 2] = 0.004                  #   (cells (4, 0) and (4, 2) see 0.004 / 0.004: not flagged)
+    rv[5, 1] = 0.004
+    rv[0, 5] = 0.0099                 # a lone 0.0099 flags nothing
+    H[2, 6] = 0.1
+    rH[2, 6] = 5.0                    # at the limit with r_H > 0: FB = 0, not flagged
+    H[3, 6] = 0.1
+    rH[3, 6] = -0.006                 # at the limit with r_H < 0: |FB| = 0.012, flagged
+    rH[0, 0] = 0.0101                 # away from the limit FB ~ r_H: flagged
+    U = O.join(np.zeros((ny, nx + 1)), np.zeros((ny + 1, nx)), H)
+    r = O.join(ru, rv, rH)
+    em = O.error_mask(ny, nx, r, U, gamma)
+    expected = np.zeros((ny, nx))
+    for c in ((1, 2), (1, 3), (4, 1), (3, 6), (0, 0)):
+        expected[c] = 1.0
+    np.testing.assert_array_equal(em, expected)
+
+
+def test_fas_fixed_point():
+    """With f = F(U*) the residual vanishes at U*, so every smoother leaves
+    U* in place and the FAS coarse problem f_c = F_c(R U*) - R r has R U*
+    as its solution: one V-cycle returns U* to rounding (a linear coarse
+    correction f_c = R r, or a sign error in f_c, moves it)."""
+    w = S.ice_sheet(32, 32, 1500.0, seed=7)
+    for bw in (1.0, 0.5):
+        h = O.Hierarchy(32, 32, 1500.0, n_levels=3)
+        h.set_schedule(pre=3, post=3, coarse=20, local_passes=2, local_sweeps=2, boundary_weight=bw)
+        h.set_geometry(w["bed"], w["H"])
+        h.set_parameters(B=w["B"], beta=w["beta"], smb=w["smb"])
+        h.forward(1.0, n_vcycles=4, update_geometry=False)     # any smooth, non-trivial state
+        U = h.field("U")
+        Ustar = U.copy()
+        F = O.residual(32, 32, 1500.0, 1.0, PRM, Ustar, np.zeros(U.size), w["bed"], w["B"], w["beta"],
+                       np.zeros((32, 32)), np.full((32, 32), 0.1))
+        h.field("f")[:] = F
+        h.vcycle()
+        scale = np.abs(Ustar).max()
+        assert np.abs(U - Ustar).max() <= 1e-10 * scale, np.abs(U - Ustar).max() / scale
+
+
+def rel_history(w, n, levels, bw, cycles):
+    h = O.Hierarchy(n, n, w["dx"], n_levels=levels)
+    h.set_schedule(boundary_weight=bw)
+    h.set_geometry(w["bed"], w["H"])
+    h.set_parameters(B=w["B"], beta=w["beta"], smb=w["smb"])
+    _, hist = h.forward(1.0, n_vcycles=cycles, tol=1e-8, history=True, update_geometry=False)
+    return hist
+
+
+def test_boundary_weight_resolves_the_edge_stall():
+    """Seeded 128^2 ice sheet (dt = 1 a, 4 levels) whose solution carries
+    normal stress on the bottom edge: with the reference's restriction the
+    relative residual stalls above 1e-6 (concentrated on the edge's v faces);
+    halving the restricted residual's boundary rows converges to 1e-8 in a
+    dozen V-cycles."""
+    w = S.ice_sheet(128, 128, 1500.0, seed=20000)
+    ref = rel_history(w, 128, 4, 1.0, 25)
+    fixed = rel_history(w, 128, 4, 0.5, 25)
+    assert len(ref) == 25 and min(ref) > 1e-6
+    assert fixed[-1] <= 1e-8 and len(fixed) <= 14
