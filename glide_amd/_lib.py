@@ -73,6 +73,7 @@ SIGNATURES = {
     "glide_vanka_adjoint_pass": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP,
                                          DP, DP, c_int, c_double, c_void_p]),
     "glide_grad_beta": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, c_int, c_void_p]),
+    "glide_grad_B": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, c_void_p]),
     "glide_restrict_u": (c_int, [DP, DP, c_int, c_int, c_void_p]),
     "glide_restrict_v": (c_int, [DP, DP, c_int, c_int, c_void_p]),
     "glide_restrict_cell": (c_int, [DP, DP, c_int, c_int, c_void_p]),

@@ -949,6 +949,40 @@ __global__ void k_adj_correct(int ny, int nx, double *__restrict__ lam, const do
     lam[k] = lam[k] + z;
 }
 
+// dL/dB = lambda . dR/dB: B enters the residual only through eta = 1/2 B
+// (...)^p, so d eta_c / d B_c = eta_c / B_c, and eta_c only through eta*H in
+// sigma_xx and sigma_yy of cell c and in the vertex average of sigma_xy at
+// its interior corners.  One thread per cell, gather form.
+__global__ void k_grad_B(GlideGeom g, Params p, double *__restrict__ grad, const double *__restrict__ U,
+                         const double *__restrict__ lam, const double *__restrict__ B)
+{
+    const int ny = g.ny, nx = g.nx;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ny * nx) return;
+    const int i = idx / nx, j = idx - i * nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv, *lu = lam, *lv = lam + nu;
+    const double dx_inv = 1.0 / g.dx;
+    const double eta = eta_cell(i, j, u, v, B, p, g.dx, ny, nx);
+    const double u_l = ld_u(u, i, j, ny, nx), u_r = ld_u(u, i, j + 1, ny, nx);
+    const double v_t = ld_v(v, i, j, ny, nx), v_b = ld_v(v, i + 1, j, ny, nx);
+    const double exx = fma(2.0 * (u_r - u_l), dx_inv, (v_t - v_b) * dx_inv);
+    const double eyy = fma(u_r - u_l, dx_inv, 2.0 * (v_t - v_b) * dx_inv);
+    double acc = 2.0 * exx * (ld_u(lu, i, j, ny, nx) - ld_u(lu, i, j + 1, ny, nx));
+    acc = fma(2.0 * eyy, ld_v(lv, i + 1, j, ny, nx) - ld_v(lv, i, j, ny, nx), acc);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int a = i + (k >> 1), b = j + (k & 1);
+        if (a < 1 || a > ny - 1 || b < 1 || b > nx - 1) continue;
+        const double exy = 0.5 * fma(ld_u(u, a - 1, b, ny, nx) - ld_u(u, a, b, ny, nx), dx_inv,
+                                     (ld_v(v, a, b, ny, nx) - ld_v(v, a, b - 1, ny, nx)) * dx_inv);
+        const double comb = (ld_u(lu, a, b, ny, nx) - ld_u(lu, a - 1, b, ny, nx)) -
+                            (ld_v(lv, a, b, ny, nx) - ld_v(lv, a, b - 1, ny, nx));
+        acc = fma(0.5 * exy, comb, acc);
+    }
+    grad[idx] = acc * H[idx] * dx_inv * (eta / B[idx]);
+}
+
 // ===========================================================================
 // Host side
 // ===========================================================================
@@ -1431,6 +1465,16 @@ int glide_grad_beta(const GlideGeom *g, const GlideParams *p, double *grad, cons
     hipLaunchKernelGGL(k_grad_beta, dim3(blocks1d(n_c(g->ny, g->nx))), dim3(256), 0, S(stream), *g, to_params(*p), grad,
                        U, lam, bed, beta, reference_edges);
     return check_launch("grad_beta");
+}
+
+int glide_grad_B(const GlideGeom *g, const GlideParams *p, double *grad, const double *U, const double *lam,
+                 const double *B, void *stream)
+{
+    TRY(check_geom(g));
+    if (!p || !grad || !U || !lam || !B) return fail(GLIDE_E_NULL, "grad_B: NULL argument");
+    hipLaunchKernelGGL(k_grad_B, dim3(blocks1d(n_c(g->ny, g->nx))), dim3(256), 0, S(stream), *g, to_params(*p), grad,
+                       U, lam, B);
+    return check_launch("grad_B");
 }
 
 int glide_adjoint_vcycle(const GlideLevel *L, int n, const GlideSchedule *sched, void *stream)

@@ -278,6 +278,22 @@ class IcePhysics:
         g.mask.zero_()
         return out
 
+    def grad_smb(self) -> torch.Tensor:
+        """dL/dsmb = -Lambda_H outside the adjoint's active set (f_H =
+        H_prev / dt + smb, glide/physics.py:173; masked rows H - gamma do not
+        depend on smb).  No reference counterpart."""
+        return self.grad_H_prev() * self.grid.dt
+
+    def grad_B(self) -> torch.Tensor:
+        """dL/dB = Lambda . dR/dB at the finest level (B enters through the
+        viscosity only: d eta / d B = eta / B).  No reference counterpart."""
+        g = self.grid
+        grad = torch.empty((g.ny, g.nx), dtype=torch.float64, device=self.device)
+        geom, prm = g.geom(), g.cparams()
+        _lib.call("glide_grad_B", ctypes.byref(geom), ctypes.byref(prm), grad.data_ptr(), g.U.data_ptr(),
+                  g.Lambda.data_ptr(), g.B.data_ptr(), self._stream())
+        return grad
+
     def adjoint_residual_norm(self) -> float:
         """||f_adj - J^T Lambda|| at the finest level with the adjoint's mask."""
         g = self.grid

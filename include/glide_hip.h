@@ -173,6 +173,14 @@ int glide_grad_beta(const GlideGeom *g, const GlideParams *p, double *grad, cons
                     const double *lam, const double *bed, const double *beta,
                     int reference_edges, void *stream);
 
+/* grad = lambda . dR/dB (B: the rate factor of the viscosity).  No reference
+ * counterpart: glide/physics.py:211-242 returns dL/dbeta only.  B enters the
+ * residual only through eta (glide/cuda/viscosity.cu:61-112), d eta / d B =
+ * eta / B, and eta through the eta*H of sigma_xx, sigma_yy and the vertex
+ * sigma_xy (glide/cuda/stress.cu:44-183). */
+int glide_grad_B(const GlideGeom *g, const GlideParams *p, double *grad, const double *U,
+                 const double *lam, const double *B, void *stream);
+
 /* Transfer operators (glide/cuda/utility.cu; launchers glide/kernels.py:90-236).
  * Sizes are the coarse (restrict) or fine (prolong) grid. */
 int glide_restrict_u(const double *uf, double *uc, int nyc, int nxc, void *stream);     /* restrict_u :1-19 */

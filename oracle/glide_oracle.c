@@ -1352,6 +1352,44 @@ void or_grad_beta(const OGeom *g, const OParams *p, double *grad, const double *
         }
 }
 
+/* dL/dB = lambda . dR/dB (no reference counterpart: glide/physics.py:211-242
+ * returns dL/dbeta only).  B enters R only through eta = 0.5 B (...)^p
+ * (viscosity.cu:61-112), so d eta_c / d B_c = eta_c / B_c, and eta_c enters
+ * only through eta*H in sigma_xx and sigma_yy of cell c (stress.cu:44-102:
+ * + on its left u face, - on its right; - on its top v face, + on its
+ * bottom) and through the vertex average eta*H_v = 1/4 sum eta_k H_k in
+ * sigma_xy of its interior corner vertices (stress.cu:160-183: vertex (a, b)
+ * enters r_u(a, b) +, r_u(a-1, b) -, r_v(a, b) -, r_v(a, b-1) +). */
+void or_grad_B(const OGeom *g, const OParams *p, double *grad, const double *U, const double *lam,
+               const double *B)
+{
+    int ny = g->ny, nx = g->nx;
+    double dx_inv = 1.0 / g->dx;
+    const double *u = U, *v = U + NU(g), *H = U + NU(g) + NV(g);
+    const double *lu = lam, *lv = lam + NU(g);
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < ny; i++)
+        for (int j = 0; j < nx; j++) {
+            double eta = eta_cell(i, j, u, v, B, p->n, p->eps_reg, g->dx, ny, nx);
+            double u_l = VF(u, i, j, ny, nx), u_r = VF(u, i, j + 1, ny, nx);
+            double v_t = HF(v, i, j, ny, nx), v_b = HF(v, i + 1, j, ny, nx);
+            double exx = fma(2.0 * (u_r - u_l), dx_inv, (v_t - v_b) * dx_inv);
+            double eyy = fma(u_r - u_l, dx_inv, 2.0 * (v_t - v_b) * dx_inv);
+            double acc = 2.0 * exx * (VF(lu, i, j, ny, nx) - VF(lu, i, j + 1, ny, nx));
+            acc = fma(2.0 * eyy, HF(lv, i + 1, j, ny, nx) - HF(lv, i, j, ny, nx), acc);
+            for (int k = 0; k < 4; k++) {
+                int a = i + (k >> 1), b = j + (k & 1);
+                if (a < 1 || a > ny - 1 || b < 1 || b > nx - 1) continue;
+                double exy = 0.5 * fma(VF(u, a - 1, b, ny, nx) - VF(u, a, b, ny, nx), dx_inv,
+                                       (HF(v, a, b, ny, nx) - HF(v, a, b - 1, ny, nx)) * dx_inv);
+                double comb = (VF(lu, a, b, ny, nx) - VF(lu, a - 1, b, ny, nx)) -
+                              (HF(lv, a, b, ny, nx) - HF(lv, a, b - 1, ny, nx));
+                acc = fma(0.5 * exy, comb, acc);
+            }
+            grad[(size_t)i * nx + j] = acc * CE(H, i, j, ny, nx) * dx_inv * (eta / CE(B, i, j, ny, nx));
+        }
+}
+
 /* ------------------------------------------------------------------ */
 /* Transfer operators (utility.cu)                                     */
 /* ------------------------------------------------------------------ */

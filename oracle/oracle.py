@@ -59,6 +59,7 @@ def _lib(libm: bool = False):
         "or_error_mask": (None, [G, DP, DP, DP, DP]),
         "or_vanka_adjoint_pass": (None, [G, P, DP, DP, DP, DP, DP, DP, DP, DP, DP, c_int, c_double, DP]),
         "or_grad_beta": (None, [G, P, DP, DP, DP, DP, DP, c_int]),
+        "or_grad_B": (None, [G, P, DP, DP, DP, DP]),
         "or_restrict_u": (None, [DP, DP, c_int, c_int]),
         "or_restrict_v": (None, [DP, DP, c_int, c_int]),
         "or_restrict_cell": (None, [DP, DP, c_int, c_int]),
@@ -228,6 +229,14 @@ def grad_beta(ny, nx, dx, dt, prm, U, lam, bed, beta, quirk=True):
     out = np.zeros(ny * nx)
     _lib().or_grad_beta(ctypes.byref(OGeom(ny, nx, dx, dt)), ctypes.byref(prm), _p(out), _p(_c(U)), _p(_c(lam)),
                         _p(_c(bed).ravel()), _p(_c(beta).ravel()), int(bool(quirk)))
+    return out.reshape(ny, nx)
+
+
+def grad_B(ny, nx, dx, prm, U, lam, B):
+    """lambda . dR/dB (B enters through the viscosity only)."""
+    out = np.zeros(ny * nx)
+    _lib().or_grad_B(ctypes.byref(OGeom(ny, nx, dx, 1.0)), ctypes.byref(prm), _p(out), _p(_c(U)), _p(_c(lam)),
+                     _p(_c(B).ravel()))
     return out.reshape(ny, nx)
 
 

@@ -134,6 +134,21 @@ def test_grad_beta_bitwise(edges):
     assert np.array_equal(o, ref), np.abs(o - ref).max()
 
 
+@pytest.mark.parametrize("shape", [(17, 23), (64, 40)])
+def test_grad_B_bitwise(shape):
+    ny, nx = shape
+    s = random_state(ny=ny, nx=nx, seed=37)
+    prm = O.params()
+    lam = np.random.default_rng(3).normal(size=s["U"].shape)
+    ref = O.grad_B(ny, nx, s["dx"], prm, s["U"], lam, s["B"])
+    g = _lib.GlideGeom(ny, nx, s["dx"], s["dt"])
+    out = torch.zeros((ny, nx), dtype=torch.float64, device=DEV)
+    _lib.call("glide_grad_B", ctypes.byref(g), ctypes.byref(cparams(prm)), P(out), P(T(s["U"])), P(T(lam)),
+              P(T(s["B"])), stream())
+    o = out.cpu().numpy()
+    assert np.array_equal(o, ref), np.abs(o - ref).max()
+
+
 def test_adjoint_vcycle_and_gradient_match_oracle():
     """Forward state from two steps, then two adjoint V-cycles and grad_beta:
     lambda and the gradient agree with the oracle to round-off propagation."""

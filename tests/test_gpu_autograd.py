@@ -79,6 +79,45 @@ def test_beta_and_Hprev_gradients_match_finite_differences():
     assert worst <= 1e-6, worst
 
 
+def test_B_and_smb_gradients_match_finite_differences():
+    """dL/dB (through the viscosity) and dL/dsmb (= -lambda_H) of the same
+    loss against fourth-order central differences of the converged solve,
+    20 random directions."""
+    w, ph = _setup(gl=False)
+    cfg = SolveConfig(dt=10.0, tol=1e-14, max_vcycles=120, adj_tol=1e-14, max_adj_vcycles=400)
+    rng = np.random.default_rng(11)
+    ny = nx = 32
+    u_obs = torch.as_tensor(rng.normal(0, 30.0, (ny, nx + 1)), device=DEV)
+    v_obs = torch.as_tensor(rng.normal(0, 30.0, (ny + 1, nx)), device=DEV)
+    wH = torch.as_tensor(rng.normal(0, 1.0, (ny, nx)), device=DEV)
+    beta = torch.as_tensor(w["beta"], device=DEV)
+    Hp = torch.as_tensor(w["H"], device=DEV)
+
+    def loss_of(B, smb):
+        u, v, H = ssa_step(ph, beta, Hp, cfg, B=B, smb=smb)
+        return 0.5 * ((u - u_obs) ** 2).sum() + 0.5 * ((v - v_obs) ** 2).sum() + (wH * H).sum()
+
+    B = torch.as_tensor(w["B"], device=DEV).clone().requires_grad_(True)
+    smb = torch.as_tensor(w["smb"], device=DEV).clone().requires_grad_(True)
+    L = loss_of(B, smb)
+    L.backward()
+    gB, gs = B.grad.detach().clone(), smb.grad.detach().clone()
+    assert ph.last_adj_history[-1] <= 1e-13
+    worst = 0.0
+    with torch.no_grad():
+        for k in range(20):
+            e_B = torch.as_tensor(rng.normal(size=(ny, nx)), device=DEV) * B.detach() * 0.1
+            e_s = torch.as_tensor(rng.normal(size=(ny, nx)), device=DEV) * (1.0 if k % 2 else 0.0)
+            h = 1e-4
+            Ls = [float(loss_of(B.detach() + c * h * e_B, smb.detach() + c * h * e_s)) for c in (1, -1, 2, -2)]
+            fd = (8.0 * (Ls[0] - Ls[1]) - (Ls[2] - Ls[3])) / (12.0 * h)
+            ad = float((gB * e_B).sum() + (gs * e_s).sum())
+            rel = abs(fd - ad) / abs(fd)
+            worst = max(worst, rel)
+            print(f"direction {k}: adjoint {ad:.10e} fd {fd:.10e} rel {rel:.2e}", flush=True)
+    assert worst <= 1e-6, worst
+
+
 def test_inversion_reduces_misfit():
     n = 32
     w, ph = _setup(n=n, gl=False)

@@ -237,3 +237,28 @@ def test_boundary_weight_resolves_the_edge_stall():
     fixed = rel_history(w, 128, 4, 0.5, 25)
     assert len(ref) == 25 and min(ref) > 1e-6
     assert fixed[-1] <= 1e-8 and len(fixed) <= 14
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_grad_B_matches_residual_finite_differences(seed):
+    """lambda . dR/dB along random directions against a fourth-order central
+    difference of lambda . R(U; B) in B (U, lambda fixed, random state with
+    thin, floating and boundary cells)."""
+    from tests.helpers import random_state
+
+    s = random_state(9, 11, seed=seed)
+    rng = np.random.default_rng(100 + seed)
+    lam = rng.normal(size=s["U"].size)
+    g = O.grad_B(s["ny"], s["nx"], s["dx"], PRM, s["U"], lam, s["B"])
+
+    def L(B):
+        r = O.residual(s["ny"], s["nx"], s["dx"], s["dt"], PRM, s["U"], s["f"], s["bed"], B, s["beta"],
+                       s["mask"], s["gamma"])
+        return float(lam @ r)
+
+    for _ in range(4):
+        d = rng.normal(size=s["B"].shape) * s["B"]
+        h = 1e-3
+        fd = (8 * (L(s["B"] + h * d) - L(s["B"] - h * d)) - (L(s["B"] + 2 * h * d) - L(s["B"] - 2 * h * d))) / (12 * h)
+        ad = float(np.sum(g * d))
+        assert abs(fd - ad) <= 1e-8 * abs(ad), (fd, ad)
