@@ -67,6 +67,8 @@ SIGNATURES = {
     "glide_vjp": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP, DP, c_void_p]),
     "glide_vanka_sweep": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP,
                                   c_int, c_int, DP, c_void_p]),
+    "glide_vanka_sweep_noalloc": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, DP,
+                                          c_int, c_int, c_void_p]),
     "glide_vanka_block": (c_int, [POINTER(GlideGeom), POINTER(GlideParams), DP, DP, DP, DP, DP, DP, c_int, DP,
                                   c_void_p]),
     "glide_local_activation": (c_int, [POINTER(GlideGeom), DP, DP, DP, DP, c_void_p]),

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "glide_hip.h"
@@ -1311,6 +1312,41 @@ int adjoint_vc(const GlideLevel *L, int n, int k, const GlideSchedule &sc, hipSt
 }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Allocation-free Vanka sweep: the per-cell updates go to a static device
+// array of 5 x 2^20 doubles (40 MiB, levels up to 1024 x 1024 cells) that
+// every GPU the library runs on holds until the process ends.  Host code
+// never passes the array's name to a kernel: its device address is looked up
+// once per device (hipGetSymbolAddress, with that device current) and kept.
+// ---------------------------------------------------------------------------
+namespace noalloc {
+constexpr long MAX_CELLS = 1L << 20;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[5 * MAX_CELLS];
+std::mutex mu;
+double *ptr[MAX_DEVICES] = {};
+
+// Device address of this device's copy of scratch.
+int address(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail((int)e, std::string("vanka_sweep_noalloc: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(GLIDE_E_ARG, "vanka_sweep_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(e != hipSuccess ? (int)e : GLIDE_E_NULL,
+                        std::string("vanka_sweep_noalloc: scratch address: ") + hipGetErrorString(e));
+        ptr[dev] = static_cast<double *>(p);
+    }
+    *out = ptr[dev];
+    return GLIDE_OK;
+}
+}  // namespace noalloc
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1401,6 +1437,22 @@ int glide_vanka_sweep(const GlideGeom *g, const GlideParams *p, double *U, const
     return launch_sweeps(*g, to_params(*p), U, f, bed, B, beta, gamma, active, n_sweeps, n_newton, scratch, S(stream));
 }
 
+
+int glide_vanka_sweep_noalloc(const GlideGeom *g, const GlideParams *p, double *U, const double *f, const double *bed,
+                              const double *B, const double *beta, const double *gamma, const unsigned char *active,
+                              int n_sweeps, int n_newton, void *stream)
+{
+    TRY(check_geom(g));
+    if (!p || !U || !f || !bed || !B || !beta || !gamma) return fail(GLIDE_E_NULL, "vanka_sweep_noalloc: NULL argument");
+    if (n_sweeps < 0 || n_newton < 0) return fail(GLIDE_E_ARG, "vanka_sweep_noalloc: negative count");
+    if ((long)g->ny * g->nx > noalloc::MAX_CELLS)
+        return fail(GLIDE_E_SIZE, "vanka_sweep_noalloc: " + std::to_string(g->ny) + " x " + std::to_string(g->nx) +
+                                      " cells exceed the static scratch (" + std::to_string(noalloc::MAX_CELLS) +
+                                      " cells)");
+    double *scratch = nullptr;
+    TRY(noalloc::address(&scratch));
+    return launch_sweeps(*g, to_params(*p), U, f, bed, B, beta, gamma, active, n_sweeps, n_newton, scratch, S(stream));
+}
 
 int glide_vanka_block(const GlideGeom *g, const GlideParams *p, const double *U, const double *f, const double *bed,
                       const double *B, const double *beta, const double *gamma, int k_updates, double *out,

@@ -139,6 +139,21 @@ int glide_vanka_sweep(const GlideGeom *g, const GlideParams *p, double *U, const
                       const double *gamma, const unsigned char *active, int n_sweeps,
                       int n_newton, double *scratch, void *stream);
 
+/* glide_vanka_sweep without the scratch argument, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): no counterpart in the reference.  The per-cell
+ * updates go to a static device array of 5 x 2^20 doubles (40 MiB) that
+ * each GPU the library runs on holds until the process ends, so levels up
+ * to 2^20 cells (1024 x 1024) are accepted and larger ones fail with
+ * GLIDE_E_SIZE.  Nothing is allocated; the array is shared by every call on
+ * a GPU, so calls that overlap on one GPU (other streams, other threads)
+ * race and must be serialised by the caller.  Same results as
+ * glide_vanka_sweep, bit for bit. */
+int glide_vanka_sweep_noalloc(const GlideGeom *g, const GlideParams *p, double *U, const double *f,
+                              const double *bed, const double *B, const double *beta,
+                              const double *gamma, const unsigned char *active, int n_sweeps,
+                              int n_newton, void *stream);
+
 /* Local 5x5 block (J, r) of every cell after k_updates damped Newton
  * updates of its 5 unknowns (out: 30 doubles per cell, J row-major then r).
  * Exposes the per-cell assembly of glide/cuda/vanka.cu:56-411 for testing. */
