@@ -34,8 +34,9 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from glide_amd.physics import TOLERANCE_SCHEDULE as TS  # noqa: E402
 from glide_amd.roofline import (FLOPS_NEWTON_ITER, PEAK_BF16_TFLOPS, PEAK_FP64_TFLOPS,  # noqa: E402
-                                PEAK_HBM_GBS, residual_bytes, sweep_flops)
+                                PEAK_HBM_GBS, linearisation_bytes, residual_bytes, sweep_flops)
 
 
 def parse():
@@ -56,6 +57,8 @@ def parse():
     ap.add_argument("--no-1024", action="store_true", help="skip the 1024^2 warm-start solves")
     ap.add_argument("--ensemble-only", action="store_true", help="only the 64-member batched ensemble")
     ap.add_argument("--only-1024", action="store_true", help="only the 1024^2 warm-start solves")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline solves and the kernel rooflines (for profiling them)")
     ap.add_argument("--cpu-report", action="store_true",
                     help="only the CPU oracle: ms per V-cycle and per step of configs 0 and 1, one thread and all")
     return ap.parse_args()
@@ -98,8 +101,21 @@ def kernel_rooflines(ph, reps=20):
                   ctypes.c_void_p(g.beta.data_ptr()), ctypes.c_void_p(g.mask.data_ptr()),
                   ctypes.c_void_p(g.gamma.data_ptr()), sp)
 
+    rng = torch.Generator(device=g.U.device).manual_seed(0)
+    dirn = torch.randn(g.U.numel(), dtype=torch.float64, device=g.U.device, generator=rng)
+    res_out = torch.empty_like(dirn)
+
+    def lin(name):
+        def f():
+            _lib.call(name, ctypes.byref(geom), ctypes.byref(prm), ctypes.c_void_p(res_out.data_ptr()),
+                      ctypes.c_void_p(g.U.data_ptr()), ctypes.c_void_p(dirn.data_ptr()),
+                      ctypes.c_void_p(g.bed.data_ptr()), ctypes.c_void_p(g.B.data_ptr()),
+                      ctypes.c_void_p(g.beta.data_ptr()), ctypes.c_void_p(g.mask.data_ptr()),
+                      ctypes.c_void_p(g.gamma.data_ptr()), sp)
+        return f
+
     out = {}
-    for name, fn in (("vanka", vanka), ("residual", resid)):
+    for name, fn in (("vanka", vanka), ("residual", resid), ("jvp", lin("glide_jvp")), ("vjp", lin("glide_vjp"))):
         fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -117,19 +133,26 @@ def kernel_rooflines(ph, reps=20):
     vanka_tf = vanka_flops / (out["vanka"] * 1e-3) / 1e12
     res_bytes = residual_bytes(g.ny, g.nx)
     res_gbs = res_bytes / (out["residual"] * 1e-3) / 1e9
+    lin_bytes = linearisation_bytes(g.ny, g.nx)
+
+    def hbm(kernel, ms, nbytes, traffic=None):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"kernel": kernel, "bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": traffic, "ms_per_launch": round(ms, 4),
+                "work": f"{nbytes} algorithmic bytes"}
+
     return {
-        "roofline": {"kernel": "k_vanka<false> + k_vanka_apply (finest full sweep, 512^2)", "bound": "mfma",
-                     "note": "FP64 VALU arithmetic (no MFMA); peak is MI355X FP64 78.6 TFLOP/s",
+        "finest_sweep_roofline": {"kernel": "k_vanka<false> + k_vanka_apply (finest full sweep, 512^2)", "bound": "valu",
+                     "note": "FP64 vector arithmetic (no MFMA); peak is MI355X FP64 78.6 TFLOP/s",
                      "achieved": round(vanka_tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(vanka_tf / PEAK_FP64_TFLOPS, 4),
                      "traffic": TRAFFIC_512["vanka"] if (g.ny, g.nx) == (512, 512) else None,
                      "ms_per_launch": round(out["vanka"], 4),
                      "work": f"{nc} cells x (eta + 10 Newton x {FLOPS_NEWTON_ITER} flop)"},
-        "stencil_roofline": {"kernel": "k_residual (512^2)", "bound": "hbm", "achieved": round(res_gbs, 1),
-                             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(res_gbs / PEAK_HBM_GBS, 4),
-                             "traffic": TRAFFIC_512["residual"] if (g.ny, g.nx) == (512, 512) else None,
-                             "ms_per_launch": round(out["residual"], 4),
-                             "work": f"{res_bytes} algorithmic bytes"},
+        "stencil_roofline": hbm("k_residual (512^2)", out["residual"], res_bytes,
+                                TRAFFIC_512["residual"] if (g.ny, g.nx) == (512, 512) else None),
+        "jvp_roofline": hbm(f"k_jvp ({g.ny}x{g.nx})", out["jvp"], lin_bytes),
+        "vjp_roofline": hbm(f"k_vjp ({g.ny}x{g.nx})", out["vjp"], lin_bytes),
     }
 
 
@@ -297,7 +320,7 @@ def ensemble_bench(dt=10.0, n_vcycles=3):
         "throughput_cell_vcycles_per_s": float(f"{cells * n_vcycles / (batched_ms * 1e-3):.4e}"),
         "autotune_s": round(tune_s, 1), "tile_heights": rec["choice"],
         "vanka_roofline": {"kernel": "kb_vanka (phase 0: 16 x 512^2 + 16 x 768^2 finest levels, one launch)",
-                           "bound": "mfma", "note": "FP64 VALU arithmetic (no MFMA)", "achieved": round(v_tf, 3),
+                           "bound": "valu", "note": "FP64 vector arithmetic (no MFMA)", "achieved": round(v_tf, 3),
                            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(v_tf / PEAK_FP64_TFLOPS, 4),
                            "ms_per_launch": round(v_ms, 4), "work": f"{v_cells} cells x (eta + 10 Newton x "
                                                                     f"{FLOPS_NEWTON_ITER} flop)", "tile_height": vty},
@@ -314,16 +337,19 @@ def ensemble_bench(dt=10.0, n_vcycles=3):
 
 WEIGHTS = os.path.join(ROOT, "glide_amd", "data", "warmstart_unet.npz")
 TOL = 1e-8
-DT = 1.0          # a: at 10 a plain FASCD stalls above 1e-8 on most 512^2 sheets of this family
+DT = 1.0
+MAX_VC = 150      # V-cycle cap of a solve to 1e-8
 N512 = 512
 DX512 = 1500.0
 LEVELS512 = 5
 
 
-def to_tolerance_bench(steps: int, warmup: int, dt=DT):
+def to_tolerance_bench(steps: int, warmup: int, dt=DT, sched=None):
     """One forward step to ||r|| <= 1e-8 ||r0_cold|| on seeded 512^2 ice
     sheets (test seeds 20000...), plain FASCD from u = v = 0 against the
-    learned warm start (network inference included in the timed region)."""
+    learned warm start (network inference included in the timed region),
+    both with the same V-cycle settings (default: the tolerance schedule)."""
+    sched = TS if sched is None else sched
     import torch
 
     from glide_amd.train_warmstart import TEST_SEED0, member
@@ -336,9 +362,10 @@ def to_tolerance_bench(steps: int, warmup: int, dt=DT):
     for k in range(warmup + steps):
         seed = TEST_SEED0 + k
         ph = member(seed, N512, dx=DX512, n_levels=LEVELS512)
+        ph.set_schedule(**sched)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        cold = plain.solve(ph, dt, tol=TOL, max_vcycles=300, use_net=False)
+        cold = plain.solve(ph, dt, tol=TOL, max_vcycles=MAX_VC, use_net=False)
         torch.cuda.synchronize()
         t_plain = (time.perf_counter() - t0) * 1e3
         row = {"seed": seed, "plain_ms": t_plain, "vcycles_plain": cold["vcycles"], "plain_converged": cold["converged"]}
@@ -347,17 +374,97 @@ def to_tolerance_bench(steps: int, warmup: int, dt=DT):
             ph.grid.v.zero_()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            warm = ws.solve(ph, dt, tol=TOL, max_vcycles=300, use_net=True)
+            warm = ws.solve(ph, dt, tol=TOL, max_vcycles=MAX_VC, use_net=True)
             torch.cuda.synchronize()
             row.update(learned_ms=(time.perf_counter() - t0) * 1e3, vcycles_learned=warm["vcycles"],
                        learned_converged=warm["converged"], fallback=warm["fallback"],
                        r0_warm_over_cold=warm["r0_warm"] / cold["r0_cold"])
         if k >= warmup:
             rows.append(row)
+            print(f"seed {seed}: plain {row['vcycles_plain']} V-cycles {t_plain:.1f} ms"
+                  + (f", learned {row['vcycles_learned']} V-cycles {row['learned_ms']:.1f} ms" if ws else ""),
+                  file=sys.stderr, flush=True)
         if last is not None:
             del last
         last = ph
     return rows, last, net
+
+
+def reference_settings_bench(seeds, dt=DT, max_vcycles=MAX_VC):
+    """Plain FASCD on the headline seeds with the reference's settings
+    (residual restriction with boundary rows weighted like interior rows,
+    200 coarsest-level sweeps): which reach 1e-8 within max_vcycles, with
+    their V-cycles and time; and with only the boundary weight changed."""
+    import numpy as np
+    import torch
+
+    from glide_amd.train_warmstart import member
+    from glide_amd.warmstart import WarmStart
+
+    plain = WarmStart(None)
+    out = {"workload": f"as the headline, plain FASCD, at most {max_vcycles} V-cycles"}
+    for name, kw in (("reference", dict(boundary_weight=1.0)), ("boundary_weight_only", dict(boundary_weight=0.5))):
+        rows = []
+        for seed in seeds:
+            ph = member(seed, N512, dx=DX512, n_levels=LEVELS512)
+            ph.set_schedule(**kw)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = plain.solve(ph, dt, tol=TOL, max_vcycles=max_vcycles, use_net=False)
+            torch.cuda.synchronize()
+            rows.append({"seed": seed, "ms": round((time.perf_counter() - t0) * 1e3, 1), "vcycles": r["vcycles"],
+                         "converged": bool(r["converged"]), "final_rel_residual": float(f"{r['history'][-1]:.3e}")})
+            del ph
+        conv = [r for r in rows if r["converged"]]
+        out[name] = {"schedule": kw, "n_converged": len(conv), "n_stalled": len(rows) - len(conv),
+                     "stalled_seeds": [r["seed"] for r in rows if not r["converged"]],
+                     "stalled_final_rel_residual": [r["final_rel_residual"] for r in rows if not r["converged"]],
+                     "converged_ms_mean": round(float(np.mean([r["ms"] for r in conv])), 2) if conv else None,
+                     "converged_vcycles": [r["vcycles"] for r in conv]}
+    return out
+
+
+def level_sweep_roofline(ph, level: int, n_sweeps: int, reps: int = 3):
+    """n_sweeps full Vanka sweeps (each a sweep launch and an apply launch)
+    on one level of the hierarchy, HIP-event timed on the solver's stream;
+    priced at the counted per-cell flops against the FP64 vector peak."""
+    import ctypes
+
+    import torch
+
+    from glide_amd import _lib
+
+    g = ph.grids[level]
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    geom, prm = g.geom(), g.cparams()
+    keep = g.U.clone()
+
+    def run():
+        _lib.call("glide_vanka_sweep", ctypes.byref(geom), ctypes.byref(prm), ctypes.c_void_p(g.U.data_ptr()),
+                  ctypes.c_void_p(g.f.data_ptr()), ctypes.c_void_p(g.bed.data_ptr()),
+                  ctypes.c_void_p(g.B.data_ptr()), ctypes.c_void_p(g.beta.data_ptr()),
+                  ctypes.c_void_p(g.gamma.data_ptr()), None, n_sweeps, 10, ctypes.c_void_p(g.scratch.data_ptr()), sp)
+
+    run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        run()
+    e1.record(st)
+    torch.cuda.synchronize()
+    g.U.copy_(keep)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / (reps * n_sweeps)
+    nc = g.ny * g.nx
+    tf = sweep_flops(nc) / (ms * 1e-3) / 1e12
+    return {"kernel": f"k_vanka_rows<false> + k_vanka_apply<false> (full sweep of the {g.ny}x{g.nx} level)",
+            "bound": "valu", "note": "FP64 vector arithmetic (no MFMA); peak = MI355X FP64 78.6 TFLOP/s; one "
+            "lane per cell, latency of the per-cell Newton chain plus two launches per sweep",
+            "achieved": round(tf, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / PEAK_FP64_TFLOPS, 5), "traffic": None, "ms_per_launch": round(ms, 5),
+            "work": f"{nc} cells x (eta + 10 Newton x {FLOPS_NEWTON_ITER} flop) = {sweep_flops(nc)} flop per sweep"}
 
 
 def learned_1024_bench(net, n_seeds=2, max_vcycles=80, dt=DT):
@@ -375,6 +482,7 @@ def learned_1024_bench(net, n_seeds=2, max_vcycles=80, dt=DT):
         row = {"seed": seed}
         for name, ws, use in (("cold", WarmStart(None), False), ("learned", WarmStart(net), True)):
             m = member(seed, 1024, dx=1000.0, n_levels=6)
+            m.set_schedule(**TS)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             r = ws.solve(m, dt, tol=1e-8, max_vcycles=max_vcycles, use_net=use)
@@ -389,7 +497,8 @@ def learned_1024_bench(net, n_seeds=2, max_vcycles=80, dt=DT):
             torch.cuda.empty_cache()
         rows.append(row)
     return {"workload": f"1024x1024, dx=1000 m, 6 levels, dt={dt:g} a, one forward step to 1e-8 (at most "
-                        f"{max_vcycles} V-cycles), cold start vs learned warm start", "seeds": rows}
+                        f"{max_vcycles} V-cycles, tolerance schedule), cold start vs learned warm start",
+            "seeds": rows}
 
 
 def learned_batched_bench(net, seeds, n_members=16, dt=DT):
@@ -401,7 +510,7 @@ def learned_batched_bench(net, seeds, n_members=16, dt=DT):
 
     from glide_amd.ensemble import Ensemble
     from glide_amd.train_warmstart import TEST_SEED0, member
-    from glide_amd.warmstart import IN_CH, N_OUT, apply_prediction, features
+    from glide_amd.warmstart import IN_CH, N_OUT, apply_prediction, features, net_scales
 
     seeds = [seeds[k % len(seeds)] for k in range(n_members)]
     out = {"members": n_members, "size": f"{N512}x{N512}", "seeds": sorted(set(seeds))}
@@ -410,6 +519,7 @@ def learned_batched_bench(net, seeds, n_members=16, dt=DT):
             break
         mem = [member(s, N512, dx=DX512, n_levels=LEVELS512) for s in seeds]
         ens = Ensemble(mem)
+        ens.set_schedule(**TS)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ens.forward_setup(dt)
@@ -420,14 +530,14 @@ def learned_batched_bench(net, seeds, n_members=16, dt=DT):
                 features(ph, out=X[k])
             pred = net.forward(X, n_members, N512, N512)
             for k, ph in enumerate(mem):
-                apply_prediction(ph, pred[k * N512 * N512 * N_OUT:(k + 1) * N512 * N512 * N_OUT])
+                apply_prediction(ph, pred[k * N512 * N512 * N_OUT:(k + 1) * N512 * N512 * N_OUT], *net_scales(net))
             rw = ens.residual_norms()
             for k, ph in enumerate(mem):       # per-member fallback
                 if not rw[k] <= r0[k]:
                     ph.grid.u.zero_()
                     ph.grid.v.zero_()
                     ph.grid.H.copy_(ph.grid.H_prev)
-        st = ens.forward(dt, tol=TOL, max_vcycles=300, update_geometry=False, r0=r0, setup=False)
+        st = ens.forward(dt, tol=TOL, max_vcycles=MAX_VC, update_geometry=False, r0=r0, setup=False)
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) * 1e3
         out[mode] = {"wall_ms": round(wall, 1), "ms_per_member_step": round(wall / n_members, 2),
@@ -534,6 +644,7 @@ def cpu_baseline_tol(seed: int, vcycles: int, threads: int, dt=DT):
         O.set_threads(threads)
     d = S.ice_sheet(N512, N512, DX512, seed)
     h = O.Hierarchy(N512, N512, DX512, n_levels=LEVELS512)
+    h.set_schedule(**TS)
     h.set_geometry(d["bed"], d["H"])
     h.set_parameters(B=d["B"], beta=d["beta"], smb=d["smb"])
     t0 = time.perf_counter()
@@ -588,31 +699,49 @@ def main():
         print(json.dumps({**kernel_rooflines(ph), "calibration_bytes": a.numel() * 8, "sum": float(s)}))
         return
     rows, ph, net = to_tolerance_bench(args.steps, args.warmup)
-    plain_ms = float(np.mean([r["plain_ms"] for r in rows]))
     learned = net is not None
-    learned_ms = float(np.mean([r["learned_ms"] for r in rows])) if learned else None
+    both = [r for r in rows if r["plain_converged"] and r.get("learned_converged", True)]
+    stalled = [r["seed"] for r in rows if not (r["plain_converged"] and r.get("learned_converged", True))]
+    plain_ms = float(np.mean([r["plain_ms"] for r in both])) if both else None
+    learned_ms = float(np.mean([r["learned_ms"] for r in both])) if (learned and both) else None
     rl = kernel_rooflines(ph)
+    # the coarsest level's 200-sweep solve: the largest kernel-time share of a headline V-cycle
+    coarse = level_sweep_roofline(ph, LEVELS512 - 1, 200)
     del ph
     torch.cuda.empty_cache()
     value = learned_ms if learned else plain_ms
+    r2 = (lambda x: round(x, 3) if x is not None else None)
     line = {
         "metric": "ms per forward step to 1e-8 relative residual at 512^2 (learned warm start vs plain FASCD), FP64",
-        "value": round(value, 3), "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(value, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "value": r2(value), "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r2(value), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded ice sheets, glide_amd/synthetic.py ice_sheet, test seeds 20000+)",
         "config": {"workload": f"ice_sheet_512: {N512}x{N512}, dx={DX512:.0f} m, {LEVELS512} levels, dt={DT:g} a, one "
                                f"forward step from u = v = 0 to ||r|| <= 1e-8 ||r0_cold||"
-                               + (" with the learned warm start" if learned else " (no trained weights: plain)"),
+                               + (" with the learned warm start" if learned else " (no trained weights: plain)")
+                               + f"; FASCD with the tolerance schedule {TS} (boundary rows of restricted "
+                               "residuals weighted 0.5: with the reference's weight 1 FASCD stalls above 1e-8 on "
+                               "sheets with normal stress at the domain edge; 50 coarsest-level sweeps instead of "
+                               "200: same V-cycle counts; see reference_settings); mean over the steps that reach "
+                               "1e-8 under both starts.  Not the 512^2 parity workload cap_512: its floating margin "
+                               "stalls FASCD near 1e-3, so it has no time to 1e-8 (its 3-V-cycle step: "
+                               "reference_schedule)",
                    "global_batch": 1, "parallelism": "single GPU"},
-        "plain_ms_per_step": round(plain_ms, 3),
-        "learned_ms_per_step": round(learned_ms, 3) if learned else None,
-        "speedup_learned_vs_plain": round(plain_ms / learned_ms, 3) if learned else None,
+        "plain_ms_per_step": r2(plain_ms),
+        "learned_ms_per_step": r2(learned_ms),
+        "speedup_learned_vs_plain": round(plain_ms / learned_ms, 3) if learned_ms else None,
+        "n_steps_converged": len(both), "n_stalled": len(stalled), "stalled_seeds": stalled,
+        "all_converged": not stalled,
         "vcycles_plain": [r["vcycles_plain"] for r in rows],
         "vcycles_learned": [r.get("vcycles_learned") for r in rows],
         "fallbacks": int(sum(bool(r.get("fallback")) for r in rows)),
-        "all_converged": bool(all(r["plain_converged"] and r.get("learned_converged", True) for r in rows)),
+        "roofline": coarse,
         **rl,
     }
+    if args.headline_only:
+        print(json.dumps(line))
+        return
+    line["reference_settings"] = reference_settings_bench([r["seed"] for r in rows])
     if learned:
         line["unet_roofline"] = unet_roofline(net)
         if not args.no_1024:

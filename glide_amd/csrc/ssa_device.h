@@ -125,9 +125,17 @@ __host__ __device__ __forceinline__ Dual eta_cell_dual(int i, int j, const doubl
     Dual exy2 = (e_tl * e_tl + e_tr * e_tr + e_bl * e_bl + e_br * e_br) * 0.25;
     Dual eII = dudx * dudx + dvdy * dvdy + dudx * dvdy + exy2 + p.eps_reg;
     double s = 0.5 * ld_c(B, i, j, ny, nx);
+    // x^p and x^(p-1) from one logarithm: dm_pow(x, q) = dm_exp(q * dm_log(x)),
+    // so the values are those of two dm_pow calls
     Dual pw;
-    pw.v = dm_pow(eII.v, glen_exp);
-    pw.d = glen_exp * dm_pow(eII.v, glen_exp - 1.0) * eII.d;
+    if (eII.v == 0.0) {   // dm_pow's own value at 0 (only reachable with eps_reg = 0)
+        pw.v = dm_pow(eII.v, glen_exp);
+        pw.d = glen_exp * dm_pow(eII.v, glen_exp - 1.0) * eII.d;
+    } else {
+        const double lg = dm_log(eII.v);
+        pw.v = dm_exp(glen_exp * lg);
+        pw.d = glen_exp * dm_exp((glen_exp - 1.0) * lg) * eII.d;
+    }
     return pw * s;
 }
 

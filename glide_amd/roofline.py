@@ -31,3 +31,12 @@ def residual_bytes(ny: int, nx: int) -> int:
     nu, nv, nc = ny * (nx + 1), (ny + 1) * nx, ny * nx
     nt = nu + nv + nc
     return 8 * (3 * nt + 5 * nc)
+
+
+def linearisation_bytes(ny: int, nx: int) -> int:
+    """Compulsory HBM bytes of one JVP or VJP: read U and the direction (or
+    lambda) (state vectors), bed, B, beta, mask (cells); write the result
+    (state vector)."""
+    nu, nv, nc = ny * (nx + 1), (ny + 1) * nx, ny * nx
+    nt = nu + nv + nc
+    return 8 * (3 * nt + 4 * nc)

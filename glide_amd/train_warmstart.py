@@ -27,8 +27,8 @@ import torch
 
 from . import synthetic as S
 from .ensemble import Ensemble
-from .physics import IcePhysics
-from .warmstart import DH_SCALE, IN_CH, N_OUT, VEL_SCALE, UNet, WarmStart, features, targets
+from .physics import TOLERANCE_SCHEDULE, IcePhysics
+from .warmstart import IN_CH, N_OUT, VEL_SCALE, UNet, WarmStart, features, targets
 
 TRAIN_SEED0 = 10000
 TEST_SEED0 = 20000
@@ -48,9 +48,13 @@ def member(seed: int, size: int, dx: float | None = None, n_levels: int | None =
 
 
 def generate(n: int, size: int, seed0: int = TRAIN_SEED0, dt: float = 1.0, tol: float = 1e-8, batch: int = 64,
-             max_vcycles: int = 80, device=None, log=print, drop_unconverged: bool = True):
-    """Solve n seeded sheets; returns (X int16 [m, size*size*32], Y fp32 [m, size*size*3], stats),
-    keeping only the m solves that reached ``tol`` unless ``drop_unconverged`` is False."""
+             max_vcycles: int = 80, device=None, log=print, drop_unconverged: bool = True,
+             hscale: float = 1.0, sched: dict | None = None):
+    """Solve n seeded sheets (batched FASCD with the tolerance schedule);
+    returns (X int16 [m, size*size*32], Y fp32 [m, size*size*3], stats),
+    keeping only the m solves that reached ``tol`` unless ``drop_unconverged``
+    is False.  Targets: cell-centred (u, v) x VEL_SCALE, (H - H_prev) x hscale."""
+    sched = TOLERANCE_SCHEDULE if sched is None else sched
     dev = torch.device(device if device is not None else "cuda")
     X = torch.empty((n, size * size * IN_CH), dtype=torch.int16, device=dev)
     Y = torch.empty((n, size * size * N_OUT), dtype=torch.float32, device=dev)
@@ -60,10 +64,11 @@ def generate(n: int, size: int, seed0: int = TRAIN_SEED0, dt: float = 1.0, tol: 
         seeds = list(range(seed0 + c0, seed0 + min(n, c0 + batch)))
         mem = [member(s, size, device=dev) for s in seeds]
         ens = Ensemble(mem)
+        ens.set_schedule(**sched)
         st = ens.forward(dt, tol=tol, max_vcycles=max_vcycles, update_geometry=False)
         for k, ph in enumerate(mem):
             features(ph, out=X[c0 + k])
-            targets(ph, out=Y[c0 + k])
+            targets(ph, out=Y[c0 + k], hscale=hscale)
         vcycles += st.vcycles
         conv += st.converged
         ens.close()
@@ -127,13 +132,16 @@ def evaluate_loss(net: UNet, X, Y, size, batch):
 
 
 def evaluate(net: UNet, seeds, size: int, dx: float | None = None, n_levels: int | None = None, dt: float = 1.0,
-             tol: float = 1e-8, max_vcycles: int = 200, device=None) -> list[dict]:
-    """Cold vs warm-started forward step on each seed: V-cycles and wall time
-    to ||r|| <= tol * r0_cold, and the prediction's error vs the cold solve."""
+             tol: float = 1e-8, max_vcycles: int = 200, device=None, sched: dict | None = None) -> list[dict]:
+    """Cold vs warm-started forward step on each seed (tolerance schedule):
+    V-cycles and wall time to ||r|| <= tol * r0_cold, and the prediction's
+    error vs the cold solve."""
+    sched = TOLERANCE_SCHEDULE if sched is None else sched
     ws = WarmStart(net)
     out = []
     for seed in seeds:
         ph = member(seed, size, dx=dx, n_levels=n_levels, device=device)
+        ph.set_schedule(**sched)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         cold = ws.solve(ph, dt, tol=tol, max_vcycles=max_vcycles, use_net=False)
@@ -192,6 +200,7 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--dt", type=float, default=1.0, help="time step of the solved forward steps (a)")
+    ap.add_argument("--dh-scale", type=float, default=1.0, help="network units of the thickness change per m")
     ap.add_argument("--n-test", type=int, default=4)
     ap.add_argument("--test-size", type=int, nargs="*", default=[256, 512, 1024])
     ap.add_argument("--out", default="warmstart_unet.npz")
@@ -202,15 +211,16 @@ def main():
         print(msg, flush=True)
 
     log(f"== generating {args.n_train} solves at {args.size}^2")
-    X, Y, gen = generate(args.n_train, args.size, tol=args.tol, dt=args.dt, log=log)
+    X, Y, gen = generate(args.n_train, args.size, tol=args.tol, dt=args.dt, log=log, hscale=args.dh_scale)
     log(f"   {gen}")
     net = UNet(seed=0)
     log(f"== training ({args.epochs} epochs, batch {args.batch})")
     tr = train(net, X, Y, args.size, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
     del X, Y
     torch.cuda.empty_cache()
-    net.save(args.out, vel_scale=VEL_SCALE, dh_scale=DH_SCALE, dt=args.dt, train_size=args.size, n_train=args.n_train, epochs=args.epochs,
-             final_val_loss=tr["history"][-1]["val_loss"])
+    net.save(args.out, vel_scale=VEL_SCALE, dh_scale=args.dh_scale, dt=args.dt, train_size=args.size,
+             n_train=args.n_train, epochs=args.epochs, final_val_loss=tr["history"][-1]["val_loss"])
+    net.meta = {"vel_scale": VEL_SCALE, "dh_scale": args.dh_scale}
     report = {"generate": gen, "train": tr}
     for size in args.test_size:
         levels = 6 if size == 1024 else (5 if size == 512 else None)

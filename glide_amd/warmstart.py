@@ -282,12 +282,26 @@ def apply_prediction(physics, pred: torch.Tensor, scale: float = VEL_SCALE, hsca
               g.v.data_ptr(), g.H.data_ptr(), g.H_prev.data_ptr(), g.gamma.data_ptr(), st)
 
 
+def net_scales(net: UNet | None) -> tuple[float, float]:
+    """(velocity, thickness-change) scales of a network's output units: the
+    values stored with its weights, else VEL_SCALE and DH_SCALE."""
+    meta = getattr(net, "meta", None) or {}
+    vs = float(meta["vel_scale"]) if "vel_scale" in meta else VEL_SCALE
+    hs = float(meta["dh_scale"]) if "dh_scale" in meta else DH_SCALE
+    return vs, hs
+
+
 class WarmStart:
     """Seeds IcePhysics forward steps with the network's (u, v) and, with
-    ``seed_thickness``, its H = max(H_prev + dH, gamma) (else H = H_prev)."""
+    ``seed_thickness``, its H = max(H_prev + dH, gamma) (else H = H_prev).
+    The output scales default to those stored with the network's weights."""
 
-    def __init__(self, net: UNet, scale: float = VEL_SCALE, hscale: float = DH_SCALE, seed_thickness: bool = True):
-        self.net, self.scale, self.hscale = net, float(scale), float(hscale)
+    def __init__(self, net: UNet, scale: float | None = None, hscale: float | None = None,
+                 seed_thickness: bool = True):
+        vs, hs = net_scales(net)
+        self.net = net
+        self.scale = vs if scale is None else float(scale)
+        self.hscale = hs if hscale is None else float(hscale)
         self.seed_thickness = bool(seed_thickness)
 
     def predict_into(self, physics):

@@ -2,6 +2,7 @@
 and what the trained network (glide_amd/data/warmstart_unet.npz) gains over
 the cold start on unseen seeds."""
 import os
+import time
 
 import numpy as np
 import pytest
@@ -13,6 +14,7 @@ if not torch.cuda.is_available():  # pragma: no cover
     pytest.skip("no GPU", allow_module_level=True)
 
 from glide_amd import synthetic as S  # noqa: E402
+from glide_amd.physics import TOLERANCE_SCHEDULE  # noqa: E402
 from glide_amd.train_warmstart import TEST_SEED0, member  # noqa: E402
 from glide_amd.warmstart import (DH_SCALE, IN_CH, N_OUT, VEL_SCALE, UNet, WarmStart,  # noqa: E402
                                  apply_prediction, features, targets)
@@ -97,32 +99,42 @@ def test_exact_seed_needs_no_vcycles_and_garbage_falls_back():
     assert bad["vcycles"] == cold["vcycles"]          # the cold start, unchanged
 
 
-def test_trained_network_saves_vcycles():
-    # unseen seeds at the training size (dt = 1 a) whose cold solve reaches
-    # 1e-8: the prediction cuts the initial residual, no fallback is taken,
-    # and the warm-started steps need fewer V-cycles in total
+def test_trained_network_saves_vcycles_and_time():
+    """The shipped network on unseen 512^2 ice sheets (dx 1500 m, 5 levels,
+    dt = 1 a, the tolerance schedule: the headline's workload, seeds past the
+    benchmark's), no seed skipped: every solve reaches 1e-8 from both starts,
+    the prediction cuts the initial residual with no fallback, and over the
+    seeds the warm start needs fewer V-cycles and less time (inference
+    included)."""
     assert os.path.exists(WEIGHTS), "trained weights are part of the product"
     net = UNet.load(WEIGHTS)
     ws = WarmStart(net)
-    saved, seeds = [], 0
-    for k in range(8):
-        ph = member(TEST_SEED0 + 10 + k, 256, n_levels=5)
-        cold = ws.solve(ph, 1.0, tol=1e-8, max_vcycles=120, use_net=False)
-        if not cold["converged"]:
-            continue                    # plain FASCD stalls on this geometry
-        ph.grid.u.zero_()
-        ph.grid.v.zero_()
-        warm = ws.solve(ph, 1.0, tol=1e-8, max_vcycles=120, use_net=True)
+    vc_cold = vc_warm = 0
+    t_cold = t_warm = 0.0
+    for k in range(6):
+        seed = TEST_SEED0 + 40 + k
+        ph = member(seed, 512, dx=1500.0, n_levels=5)
+        ph.set_schedule(**TOLERANCE_SCHEDULE)
+        cold = ws.solve(ph, 1.0, tol=1e-8, max_vcycles=100, use_net=False)     # warm-up of this seed's buffers
+        for rep in range(2):
+            ph.grid.u.zero_()
+            ph.grid.v.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = ws.solve(ph, 1.0, tol=1e-8, max_vcycles=100, use_net=bool(rep))
+            torch.cuda.synchronize()
+            dt_ms = (time.perf_counter() - t0) * 1e3
+            if rep == 0:
+                cold, tc = r, dt_ms
+            else:
+                warm, tw = r, dt_ms
         ratio = warm["r0_warm"] / cold["r0_cold"]
-        print(f"seed {TEST_SEED0 + 10 + k}: cold {cold['vcycles']} warm {warm['vcycles']} r0 ratio {ratio:.3e}",
-              flush=True)
-        assert warm["converged"] and not warm["fallback"] and ratio < 0.5
-        saved.append(cold["vcycles"] - warm["vcycles"])
-        seeds += 1
-        if seeds == 4:
-            break
-    assert seeds >= 3
-    # per seed the V-cycle count of FASCD from a near-solution start varies
-    # (0, 7 and 17 saved on the first three converging seeds); in total the
-    # warm start saves V-cycles and no seed needs more than two extra
-    assert sum(saved) >= 3 and min(saved) >= -2, saved
+        print(f"seed {seed}: cold {cold['vcycles']} V-cycles {tc:.1f} ms, warm {warm['vcycles']} V-cycles "
+              f"{tw:.1f} ms, r0 ratio {ratio:.3e}", flush=True)
+        assert cold["converged"] and warm["converged"] and not warm["fallback"] and ratio < 0.5
+        vc_cold += cold["vcycles"]
+        vc_warm += warm["vcycles"]
+        t_cold += tc
+        t_warm += tw
+    print(f"total: cold {vc_cold} V-cycles {t_cold:.1f} ms, warm {vc_warm} V-cycles {t_warm:.1f} ms", flush=True)
+    assert vc_warm < vc_cold and t_warm < t_cold
