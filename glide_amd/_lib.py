@@ -136,6 +136,7 @@ SIGNATURES = {
                                             c_void_p]),
     "glide_set_persistence": (c_int, [c_int, c_int]),
     "glide_set_row_split": (c_int, [ctypes.c_long]),
+    "glide_set_tile_height": (c_int, [c_int, c_int]),
     "glide_count_nonfinite": (c_int, [DP, ctypes.c_long, c_void_p, POINTER(ctypes.c_ulonglong), c_void_p]),
     "glide_vti_block_bytes": (ctypes.c_size_t, [POINTER(GlideVtiArray), c_int, c_int, c_int]),
     "glide_vti_pack": (c_int, [POINTER(GlideVtiArray), c_int, c_int, c_int, c_int, c_void_p, ctypes.c_size_t,

@@ -62,9 +62,23 @@ inline hipStream_t S(void *s) { return (hipStream_t)s; }
 // hardware then balances clustered local-smoother work); the residual's 4
 // loops each workgroup over several tiles.  Launch shapes only: results do
 // not depend on them.
-std::atomic<int> g_wg_per_cu[2] = {{64}, {4}};
+std::atomic<int> g_wg_per_cu[2] = {{64}, {8}};
 // Levels with at most this many cells use the row-split Vanka sweep.
 std::atomic<long> g_row_split_max_cells{128 * 128};
+// Tile heights (64 x TY points) of the single-level residual (4, 8, 16; 0 =
+// by level size) and of the JVP / VJP (0 = the per-point form, 4, 8); set by
+// glide_set_tile_height.  Launch shapes only: results are the same bits.
+// Measured at 512^2: residual 64 x 16 tiles 11.3 us, 64 x 8 12.0, 64 x 4 13.4;
+// at 1024^2: 64 x 8 39.7 us, 64 x 16 42.0, 64 x 4 46.4 (8 workgroups per CU).
+std::atomic<int> g_res_ty{0};
+std::atomic<int> g_lin_ty{8};
+int residual_ty(int ny, int nx)
+{
+    const int ty = g_res_ty;
+    if (ty) return ty;
+    const long nc = (long)ny * nx;
+    return nc > (1L << 18) ? 8 : (nc >= (1L << 16) ? 16 : 4);
+}
 
 int n_cu()
 {
@@ -85,7 +99,7 @@ dim3 persistent_grid(long tiles, int per_cu)
     return dim3((unsigned)((g + NXCD - 1) / NXCD * NXCD));
 }
 dim3 vanka_grid(int ny, int nx) { return persistent_grid((long)cdiv(nx, TX) * cdiv(ny, TY), g_wg_per_cu[0]); }
-dim3 residual_grid(int ny, int nx) { return persistent_grid((long)cdiv(nx, TX) * cdiv(ny, TY), g_wg_per_cu[1]); }
+dim3 residual_grid(int ny, int nx, int ty) { return persistent_grid((long)cdiv(nx, TX) * cdiv(ny, ty), g_wg_per_cu[1]); }
 dim3 point_grid(int ny, int nx) { return dim3((nx + 1 + TX - 1) / TX, (ny + 1 + TY - 1) / TY); }
 dim3 cell_grid(int ny, int nx) { return dim3((nx + TX - 1) / TX, (ny + TY - 1) / TY); }
 inline unsigned blocks1d(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
@@ -98,17 +112,19 @@ int glide::set_error(int code, const char *msg) { return fail(code, msg); }
 // ---------------------------------------------------------------------------
 // Residual r = F(U) - f (- sub)  (glide/cuda/residuals.cu:6-292)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_residual(GlideGeom g, Params p, double *__restrict__ r,
-                                                 const double *__restrict__ U, const double *__restrict__ f,
-                                                 const double *__restrict__ sub, const double *__restrict__ bed,
-                                                 const double *__restrict__ B, const double *__restrict__ beta,
-                                                 const double *__restrict__ mask, const double *__restrict__ gamma)
+template <int RTY>
+__global__ __launch_bounds__(TX *RTY) void k_residual(GlideGeom g, Params p, double *__restrict__ r,
+                                                      const double *__restrict__ U, const double *__restrict__ f,
+                                                      const double *__restrict__ sub, const double *__restrict__ bed,
+                                                      const double *__restrict__ B, const double *__restrict__ beta,
+                                                      const double *__restrict__ mask,
+                                                      const double *__restrict__ gamma)
 {
-    __shared__ ResidLDS<TX, TY> S;
-    const int tx = cdiv(g.nx, TX), total = tx * cdiv(g.ny, TY);   // cell tiles; edge faces by the last tiles
+    __shared__ ResidLDS<TX, RTY> S;
+    const int tx = cdiv(g.nx, TX), total = tx * cdiv(g.ny, RTY);   // cell tiles; edge faces by the last tiles
     tile_loop(total, [&](int t) {
         __syncthreads();   // the previous tile's readers of S are done
-        residual_tile_lds<TX, TY, true>(g, p, r, U, f, sub, bed, B, beta, mask, gamma, t % tx, t / tx, S);
+        residual_tile_lds<TX, RTY, true>(g, p, r, U, f, sub, bed, B, beta, mask, gamma, t % tx, t / tx, S);
     });
 }
 
@@ -650,7 +666,7 @@ __device__ __forceinline__ double jvp_v_row(int i, int j, int ny, int nx, double
     return d;
 }
 
-__global__ __launch_bounds__(NT) void k_jvp(GlideGeom g, Params p, double *__restrict__ jo, const double *__restrict__ U,
+__global__ __launch_bounds__(NT) void k_jvp_point(GlideGeom g, Params p, double *__restrict__ jo, const double *__restrict__ U,
                                             const double *__restrict__ dU, const double *__restrict__ bed,
                                             const double *__restrict__ B, const double *__restrict__ beta,
                                             const double *__restrict__ mask)
@@ -707,7 +723,7 @@ __device__ __forceinline__ double vertex_deps(int a, int b, int ny, int nx, doub
 
 // l = J^T lambda in gather form; out = base - l when base != NULL
 // (the adjoint residual r_adj = f_adj - J^T lambda, glide/grid.py:322-324)
-__global__ __launch_bounds__(NT) void k_vjp(GlideGeom g, Params p, double *__restrict__ out, const double *__restrict__ base,
+__global__ __launch_bounds__(NT) void k_vjp_point(GlideGeom g, Params p, double *__restrict__ out, const double *__restrict__ base,
                                             const double *__restrict__ U, const double *__restrict__ lam,
                                             const double *__restrict__ bed, const double *__restrict__ B,
                                             const double *__restrict__ beta, const double *__restrict__ mask)
@@ -819,6 +835,27 @@ __global__ __launch_bounds__(NT) void k_vjp(GlideGeom g, Params p, double *__res
         }
         emit(nu + nv + (size_t)i * nx + j, acc);
     }
+}
+
+// LDS-staged JVP and VJP on 64 x LTY point tiles (lin_tile_lds)
+template <int LTY>
+__global__ __launch_bounds__(TX *LTY) void k_jvp(GlideGeom g, Params p, double *__restrict__ jo,
+                                                 const double *__restrict__ U, const double *__restrict__ dU,
+                                                 const double *__restrict__ bed, const double *__restrict__ B,
+                                                 const double *__restrict__ beta, const double *__restrict__ mask)
+{
+    __shared__ LinLDS<TX, LTY> S;
+    lin_tile_lds<TX, LTY, false>(g, p, jo, nullptr, U, dU, bed, B, beta, mask, blockIdx.x, blockIdx.y, S);
+}
+template <int LTY>
+__global__ __launch_bounds__(TX *LTY) void k_vjp(GlideGeom g, Params p, double *__restrict__ out,
+                                                 const double *__restrict__ base, const double *__restrict__ U,
+                                                 const double *__restrict__ lam, const double *__restrict__ bed,
+                                                 const double *__restrict__ B, const double *__restrict__ beta,
+                                                 const double *__restrict__ mask)
+{
+    __shared__ LinLDS<TX, LTY> S;
+    lin_tile_lds<TX, LTY, true>(g, p, out, base, U, lam, bed, B, beta, mask, blockIdx.x, blockIdx.y, S);
 }
 
 // Adjoint red-black Vanka pass, in place: cells of one colour own disjoint
@@ -993,8 +1030,16 @@ int launch_residual(const GlideGeom &g, const Params &p, double *r, const double
                     const double *bed, const double *B, const double *beta, const double *mask, const double *gamma,
                     hipStream_t st)
 {
-    hipLaunchKernelGGL(k_residual, residual_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, r, U, f, sub, bed, B, beta,
-                       mask, gamma);
+    const int ty = residual_ty(g.ny, g.nx);
+    if (ty == 16)
+        hipLaunchKernelGGL(k_residual<16>, residual_grid(g.ny, g.nx, 16), dim3(TX, 16), 0, st, g, p, r, U, f, sub, bed,
+                           B, beta, mask, gamma);
+    else if (ty == 8)
+        hipLaunchKernelGGL(k_residual<8>, residual_grid(g.ny, g.nx, 8), dim3(TX, 8), 0, st, g, p, r, U, f, sub, bed, B,
+                           beta, mask, gamma);
+    else
+        hipLaunchKernelGGL(k_residual<4>, residual_grid(g.ny, g.nx, 4), dim3(TX, 4), 0, st, g, p, r, U, f, sub, bed, B,
+                           beta, mask, gamma);
     return check_launch("residual");
 }
 
@@ -1243,7 +1288,15 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
 int launch_vjp(const GlideGeom &g, const Params &p, double *out, const double *base, const double *U, const double *lam,
                const double *bed, const double *B, const double *beta, const double *mask, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_vjp, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, out, base, U, lam, bed, B, beta, mask);
+    const int ty = g_lin_ty;
+    const dim3 grid4(cdiv(g.nx + 1, TX), cdiv(g.ny + 1, 4)), grid8(cdiv(g.nx + 1, TX), cdiv(g.ny + 1, 8));
+    if (ty == 8)
+        hipLaunchKernelGGL(k_vjp<8>, grid8, dim3(TX, 8), 0, st, g, p, out, base, U, lam, bed, B, beta, mask);
+    else if (ty == 4)
+        hipLaunchKernelGGL(k_vjp<4>, grid4, dim3(TX, 4), 0, st, g, p, out, base, U, lam, bed, B, beta, mask);
+    else
+        hipLaunchKernelGGL(k_vjp_point, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, out, base, U, lam, bed, B,
+                           beta, mask);
     return check_launch("vjp");
 }
 
@@ -1408,6 +1461,21 @@ int glide_set_row_split(long max_cells)
     return GLIDE_OK;
 }
 
+int glide_set_tile_height(int kernel, int ty)
+{
+    if (kernel == 0) {
+        if (ty != 0 && ty != 4 && ty != 8 && ty != 16)
+            return fail(GLIDE_E_ARG, "residual tile height must be 0 (by level size), 4, 8 or 16");
+        g_res_ty = ty;
+    } else if (kernel == 1) {
+        if (ty != 0 && ty != 4 && ty != 8) return fail(GLIDE_E_ARG, "JVP/VJP tile height must be 0, 4 or 8");
+        g_lin_ty = ty;
+    } else {
+        return fail(GLIDE_E_ARG, "kernel must be 0 (residual) or 1 (JVP/VJP)");
+    }
+    return GLIDE_OK;
+}
+
 int glide_set_persistence(int kernel, int wg_per_cu)
 {
     if (kernel < 0 || kernel > 1) return fail(GLIDE_E_ARG, "kernel must be 0 (Vanka sweep) or 1 (residual)");
@@ -1481,7 +1549,14 @@ int glide_jvp(const GlideGeom *g, const GlideParams *p, double *j, const double 
     TRY(check_geom(g));
     if (!p || !j || !U || !dU || !bed || !B || !beta || !mask) return fail(GLIDE_E_NULL, "jvp: NULL argument");
     (void)gamma;
-    hipLaunchKernelGGL(k_jvp, point_grid(g->ny, g->nx), dim3(TX, TY), 0, S(stream), *g, to_params(*p), j, U, dU, bed,
+    const int ty = g_lin_ty;
+    const dim3 grid4(cdiv(g->nx + 1, TX), cdiv(g->ny + 1, 4)), grid8(cdiv(g->nx + 1, TX), cdiv(g->ny + 1, 8));
+    if (ty == 8)
+        hipLaunchKernelGGL(k_jvp<8>, grid8, dim3(TX, 8), 0, S(stream), *g, to_params(*p), j, U, dU, bed, B, beta, mask);
+    else if (ty == 4)
+        hipLaunchKernelGGL(k_jvp<4>, grid4, dim3(TX, 4), 0, S(stream), *g, to_params(*p), j, U, dU, bed, B, beta, mask);
+    else
+    hipLaunchKernelGGL(k_jvp_point, point_grid(g->ny, g->nx), dim3(TX, TY), 0, S(stream), *g, to_params(*p), j, U, dU, bed,
                        B, beta, mask);
     return check_launch("jvp");
 }

@@ -101,16 +101,14 @@ __host__ __device__ __forceinline__ double eta_cell(int i, int j, const double *
 }
 
 // Dual-number viscosity: value and directional derivative along (du, dv)
-// (viscosity.cu:4-59).
-__host__ __device__ __forceinline__ Dual eta_cell_dual(int i, int j, const double *__restrict__ u,
-                                              const double *__restrict__ v, const double *__restrict__ du,
-                                              const double *__restrict__ dv, const double *__restrict__ B,
-                                              const Params &p, double dx, int ny, int nx)
+// (viscosity.cu:4-59).  U(a, b) and V(a, b) return the clamped Dual
+// (u[a, b], du[a, b]) and (v[a, b], dv[a, b]); Bc is B at the cell.
+template <class UA, class VA>
+__host__ __device__ __forceinline__ Dual eta_cell_dual_acc(int i, int j, const UA &U, const VA &V, double Bc,
+                                                           const Params &p, double dx, int ny, int nx)
 {
     double dx_inv = 1.0 / dx;
     double glen_exp = (1.0 - p.n) / (2.0 * p.n);
-    auto U = [&](int a, int b) { return Dual{ld_u(u, a, b, ny, nx), ld_u(du, a, b, ny, nx)}; };
-    auto V = [&](int a, int b) { return Dual{ld_v(v, a, b, ny, nx), ld_v(dv, a, b, ny, nx)}; };
     Dual u_l = U(i, j), u_r = U(i, j + 1), v_t = V(i, j), v_b = V(i + 1, j);
     Dual dudx = (u_r - u_l) * dx_inv;
     Dual dvdy = (v_t - v_b) * dx_inv;
@@ -124,7 +122,7 @@ __host__ __device__ __forceinline__ Dual eta_cell_dual(int i, int j, const doubl
     Dual e_br = (((u_r - U(i + 1, j + 1)) * dx_inv + (V(i + 1, j + 1) - v_b) * dx_inv) * 0.5) * br_mask;
     Dual exy2 = (e_tl * e_tl + e_tr * e_tr + e_bl * e_bl + e_br * e_br) * 0.25;
     Dual eII = dudx * dudx + dvdy * dvdy + dudx * dvdy + exy2 + p.eps_reg;
-    double s = 0.5 * ld_c(B, i, j, ny, nx);
+    double s = 0.5 * Bc;
     // x^p and x^(p-1) from one logarithm: dm_pow(x, q) = dm_exp(q * dm_log(x)),
     // so the values are those of two dm_pow calls
     Dual pw;
@@ -137,6 +135,15 @@ __host__ __device__ __forceinline__ Dual eta_cell_dual(int i, int j, const doubl
         pw.d = glen_exp * dm_exp((glen_exp - 1.0) * lg) * eII.d;
     }
     return pw * s;
+}
+__host__ __device__ __forceinline__ Dual eta_cell_dual(int i, int j, const double *__restrict__ u,
+                                              const double *__restrict__ v, const double *__restrict__ du,
+                                              const double *__restrict__ dv, const double *__restrict__ B,
+                                              const Params &p, double dx, int ny, int nx)
+{
+    auto U = [&](int a, int b) { return Dual{ld_u(u, a, b, ny, nx), ld_u(du, a, b, ny, nx)}; };
+    auto V = [&](int a, int b) { return Dual{ld_v(v, a, b, ny, nx), ld_v(dv, a, b, ny, nx)}; };
+    return eta_cell_dual_acc(i, j, U, V, ld_c(B, i, j, ny, nx), p, dx, ny, nx);
 }
 
 // eta*H at a vertex from its four cells (viscosity.cu:201-217)

@@ -469,4 +469,285 @@ __device__ __forceinline__ double prol_H_at(const double *hc, const double *wc, 
                  cdiff(hc, wc, Jh * s + Ih));
 }
 
+// ---------------------------------------------------------------------------
+// Linearisation, LDS-staged like the residual: JVP j = J(U) d
+// (glide/cuda/residuals.cu:299-579) and the gather-form VJP l = J(U)^T lambda
+// (residuals.cu:586-1023; the (u, v) block through its symmetry, as a JVP
+// along lambda with a zero thickness direction).  The tile's u, v and the
+// direction (two-ring halo) and H, bed, beta and the thickness direction (or
+// lambda_H (1 - mask)) (one-ring halo) are staged once; each cell's dual
+// viscosity, grounding sigmoid and the directional derivatives of sigma_xx
+// and sigma_yy, and each vertex's of sigma_xy, are evaluated once; each point
+// then sums its terms from LDS.  Every quantity is the expression of the
+// per-point form on the same clamped values and every output sums its terms
+// in the same order, so the bits are those of the per-point form.
+// Tiles cover the points [0, ny] x [0, nx].
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+struct LinLDS {
+    double u[TY + 4][TX + 4], v[TY + 4][TX + 4];     // state, clamped, global (i0-2+a, j0-2+b)
+    double du[TY + 4][TX + 4], dv[TY + 4][TX + 4];   // direction (JVP) or lambda (VJP), clamped
+    double H[TY + 2][TX + 2], dH[TY + 2][TX + 2];    // cells (i0-1+a, j0-1+b), clamped; dH: direction or lambda_H (1 - mask)
+    double bed[TY + 2][TX + 2], beta[TY + 2][TX + 2];
+    double ev[TY + 2][TX + 2], ed[TY + 2][TX + 2];   // dual viscosity (0 outside the domain)
+    double g[TY + 2][TX + 2], gd[TY + 2][TX + 2];    // grounding sigmoid and its derivative
+    double txx[TY + 2][TX + 2], tyy[TY + 2][TX + 2]; // directional derivatives of sigma_xx, sigma_yy
+    double txy[TY + 1][TX + 1];                      // ... of sigma_xy, vertices (i0+a, j0+b)
+    double exy[TY + 1][TX + 1];                      // 2 eps_xy at the vertices (0 on the boundary); VJP only
+};
+
+// eta*H as a Dual: cell and vertex average (viscosity.cu:150-217)
+__device__ __forceinline__ Dual eH_cell_dual(Dual eta, Dual Hh) { return Dual{Hh.v * eta.v, Hh.v * eta.d + eta.v * Hh.d}; }
+__device__ __forceinline__ Dual eH_vertex_dual(Dual a, Dual b, Dual c, Dual d, Dual Ha, Dual Hb, Dual Hc, Dual Hdd)
+{
+    return Dual{etaH_vertex(a.v, b.v, c.v, d.v, Ha.v, Hb.v, Hc.v, Hdd.v),
+                0.25 * Ha.v * a.d + 0.25 * a.v * Ha.d + 0.25 * Hb.v * b.d + 0.25 * b.v * Hb.d + 0.25 * Hc.v * c.d +
+                    0.25 * c.v * Hc.d + 0.25 * Hdd.v * d.d + 0.25 * d.v * Hdd.d};
+}
+
+// VJP: out = base - l when base != NULL (the adjoint residual f_adj - J^T lambda)
+template <int TX, int TY, bool VJP>
+__device__ __forceinline__ void lin_tile_lds(const GlideGeom &g, const Params &p, double *__restrict__ out,
+                                             const double *__restrict__ base, const double *__restrict__ U,
+                                             const double *__restrict__ D, const double *__restrict__ bed,
+                                             const double *__restrict__ B, const double *__restrict__ beta,
+                                             const double *__restrict__ mask, int bx, int by, LinLDS<TX, TY> &S)
+{
+    constexpr int NT = TX * TY;
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv;
+    const double *du = D, *dv = D + nu, *dH = D + nu + nv;
+    const int i0 = by * TY, j0 = bx * TX;
+    const int tid = threadIdx.y * TX + threadIdx.x;
+    const double dx_inv = 1.0 / g.dx, dt = g.dt;
+    // 1. stage the inputs
+    for (int k = tid; k < (TY + 4) * (TX + 4); k += NT) {
+        const int a = k / (TX + 4), b = k - a * (TX + 4);
+        const int gi = i0 - 2 + a, gj = j0 - 2 + b;
+        S.u[a][b] = ld_u(u, gi, gj, ny, nx);
+        S.v[a][b] = ld_v(v, gi, gj, ny, nx);
+        S.du[a][b] = ld_u(du, gi, gj, ny, nx);
+        S.dv[a][b] = ld_v(dv, gi, gj, ny, nx);
+    }
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        const int a = k / (TX + 2), b = k - a * (TX + 2);
+        const int ci = i0 - 1 + a, cj = j0 - 1 + b;
+        S.H[a][b] = ld_c(H, ci, cj, ny, nx);
+        S.bed[a][b] = ld_c(bed, ci, cj, ny, nx);
+        S.beta[a][b] = ld_c(beta, ci, cj, ny, nx);
+        S.dH[a][b] = VJP ? ld_c(dH, ci, cj, ny, nx) * (1.0 - ld_c(mask, ci, cj, ny, nx)) : ld_c(dH, ci, cj, ny, nx);
+    }
+    __syncthreads();
+    auto UL = [&](int a, int b) { return S.u[a - i0 + 2][b - j0 + 2]; };   // global (a, b)
+    auto VL = [&](int a, int b) { return S.v[a - i0 + 2][b - j0 + 2]; };
+    auto DU = [&](int a, int b) { return S.du[a - i0 + 2][b - j0 + 2]; };
+    auto DV = [&](int a, int b) { return S.dv[a - i0 + 2][b - j0 + 2]; };
+    auto Ud = [&](int a, int b) { return Dual{UL(a, b), DU(a, b)}; };
+    auto Vd = [&](int a, int b) { return Dual{VL(a, b), DV(a, b)}; };
+    auto HC = [&](int a, int b) { return S.H[a - i0 + 1][b - j0 + 1]; };
+    auto LM = [&](int a, int b) { return S.dH[a - i0 + 1][b - j0 + 1]; };   // VJP: lambda_H (1 - mask)
+    auto Hd = [&](int a, int b) { return Dual{HC(a, b), VJP ? 0.0 : S.dH[a - i0 + 1][b - j0 + 1]}; };
+    auto E = [&](int a, int b) { return Dual{S.ev[a - i0 + 1][b - j0 + 1], S.ed[a - i0 + 1][b - j0 + 1]}; };
+    auto G = [&](int a, int b) { return Ground{S.g[a - i0 + 1][b - j0 + 1], S.gd[a - i0 + 1][b - j0 + 1]}; };
+    auto BED = [&](int a, int b) { return S.bed[a - i0 + 1][b - j0 + 1]; };
+    auto BETA = [&](int a, int b) { return S.beta[a - i0 + 1][b - j0 + 1]; };
+    // 2. per cell: dual viscosity and grounding sigmoid
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        const int a = k / (TX + 2), b = k - a * (TX + 2);
+        const int ci = i0 - 1 + a, cj = j0 - 1 + b;
+        const Dual e = (ci >= 0 && ci < ny && cj >= 0 && cj < nx)
+                           ? eta_cell_dual_acc(ci, cj, Ud, Vd, B[(size_t)ci * nx + cj], p, g.dx, ny, nx)
+                           : Dual{0.0, 0.0};
+        S.ev[a][b] = e.v;
+        S.ed[a][b] = e.d;
+        const Ground gr = ground(S.bed[a][b], S.H[a][b], p);
+        S.g[a][b] = gr.g;
+        S.gd[a][b] = gr.gd;
+    }
+    __syncthreads();
+    // 3. directional derivatives of the stresses: sigma_xx on cells [i0, i0+TY) x
+    //    [j0-1, j0+TX), sigma_yy on [i0-1, i0+TY) x [j0, j0+TX), sigma_xy on the vertices
+    for (int k = tid; k < (TY + 2) * (TX + 2); k += NT) {
+        const int a = k / (TX + 2), b = k - a * (TX + 2);
+        const int ci = i0 - 1 + a, cj = j0 - 1 + b;
+        double tx = 0.0, ty = 0.0;
+        if ((a >= 1 && a <= TY && b <= TX) || (a <= TY && b >= 1 && b <= TX)) {
+            const Dual eH = eH_cell_dual(E(ci, cj), Hd(ci, cj));
+            const Dual ul = Ud(ci, cj), ur = Ud(ci, cj + 1), vt = Vd(ci, cj), vb = Vd(ci + 1, cj);
+            if (a >= 1 && a <= TY && b <= TX) {
+                const SigN s = sigma_xx(ul.v, ur.v, vt.v, vb.v, eH.v, dx_inv, cj, nx);
+                tx = sigN_jvp(s, ul.d, ur.d, vt.d, vb.d, eH.d);
+            }
+            if (a <= TY && b >= 1 && b <= TX) {
+                const SigN s = sigma_yy(ul.v, ur.v, vt.v, vb.v, eH.v, dx_inv, ci, ny);
+                ty = sigN_jvp(s, ul.d, ur.d, vt.d, vb.d, eH.d);
+            }
+        }
+        S.txx[a][b] = tx;
+        S.tyy[a][b] = ty;
+    }
+    for (int k = tid; k < (TY + 1) * (TX + 1); k += NT) {
+        const int a = k / (TX + 1), b = k - a * (TX + 1);
+        const int vi = i0 + a, vj = j0 + b;
+        const Dual eH = eH_vertex_dual(E(vi - 1, vj - 1), E(vi - 1, vj), E(vi, vj - 1), E(vi, vj), Hd(vi - 1, vj - 1),
+                                       Hd(vi - 1, vj), Hd(vi, vj - 1), Hd(vi, vj));
+        const Dual ut = Ud(vi - 1, vj), ub = Ud(vi, vj), vl = Vd(vi, vj - 1), vr = Vd(vi, vj);
+        const SigS s = sigma_xy(ut.v, ub.v, vl.v, vr.v, eH.v, dx_inv, vi, vj, ny, nx);
+        S.txy[a][b] = sigS_jvp(s, ut.d, ub.d, vl.d, vr.d, eH.d);
+        if (VJP) S.exy[a][b] = s.d_eta_H;
+    }
+    __syncthreads();
+    auto TXX = [&](int ci, int cj) { return S.txx[ci - i0 + 1][cj - j0 + 1]; };
+    auto TYY = [&](int ci, int cj) { return S.tyy[ci - i0 + 1][cj - j0 + 1]; };
+    auto TXY = [&](int vi, int vj) { return S.txy[vi - i0][vj - j0]; };
+    auto EXY = [&](int vi, int vj) { return S.exy[vi - i0][vj - j0]; };
+    const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
+    if (i > ny || j > nx) return;
+    auto emit = [&](size_t k, double val) { out[k] = (VJP && base) ? base[k] - val : val; };
+    // 4a. u row (i, j): J_u d (glide/cuda/residuals.cu:378-499)
+    if (i < ny) {
+        double d = 0.0;
+        d += TXX(i, j) * dx_inv;
+        d -= TXX(i, j - 1) * dx_inv;
+        d += TXY(i, j) * dx_inv;
+        d -= TXY(i + 1, j) * dx_inv;
+        const Dual ul = Ud(i, j), Hl = Hd(i, j - 1), Hc = Hd(i, j);
+        {
+            const Ground gl = G(i, j - 1), gc = G(i, j);
+            TauB t = tau_b_g(ul.v, gl.g, gc.g, gl.gd, gc.gd, BETA(i, j - 1), BETA(i, j), p);
+            d += t.d_u * ul.d + t.d_H_l * Hl.d + t.d_H_r * Hc.d;
+        }
+        {
+            TauD t = tau_dx(Hl.v, Hc.v, BED(i, j - 1), BED(i, j), dx_inv, j, nx);
+            d -= t.d_H_l * Hl.d + t.d_H_r * Hc.d;
+        }
+        if (VJP) {   // plus the mass rows through the fluxes of this face
+            Flux q = flux_x(ul.v, HC(i, j - 1), HC(i, j), j, nx);
+            double acc = 0.0;
+            if (j > 0) acc += LM(i, j - 1) * q.d_u * dx_inv;
+            if (j < nx) acc += -LM(i, j) * q.d_u * dx_inv;
+            acc += d;
+            d = acc;
+        }
+        emit((size_t)i * (nx + 1) + j, d);
+    }
+    // 4b. v row (i, j) (residuals.cu:501-576)
+    if (j < nx) {
+        double d = 0.0;
+        d += TYY(i - 1, j) * dx_inv;
+        d -= TYY(i, j) * dx_inv;
+        d -= TXY(i, j) * dx_inv;
+        d += TXY(i, j + 1) * dx_inv;
+        const Dual vt = Vd(i, j), Ht = Hd(i - 1, j), Hc = Hd(i, j);
+        {
+            const Ground gt = G(i - 1, j), gc = G(i, j);
+            TauB t = tau_b_g(vt.v, gt.g, gc.g, gt.gd, gc.gd, BETA(i - 1, j), BETA(i, j), p);
+            d += t.d_u * vt.d + t.d_H_l * Ht.d + t.d_H_r * Hc.d;
+        }
+        {
+            TauD t = tau_dy(Ht.v, Hc.v, BED(i - 1, j), BED(i, j), dx_inv, i, ny);
+            d -= t.d_H_l * Ht.d + t.d_H_r * Hc.d;
+        }
+        if (VJP) {
+            Flux q = flux_y(vt.v, HC(i - 1, j), HC(i, j), i, ny);
+            double acc = 0.0;
+            if (i > 0) acc += -LM(i - 1, j) * q.d_u * dx_inv;
+            if (i < ny) acc += LM(i, j) * q.d_u * dx_inv;
+            acc += d;
+            d = acc;
+        }
+        emit(nu + (size_t)i * nx + j, d);
+    }
+    if (i >= ny || j >= nx) return;
+    const size_t kc = (size_t)i * nx + j;
+    if (!VJP) {   // 4c. mass row of cell (i, j) (residuals.cu:343-375)
+        const Dual H_c = Hd(i, j);
+        double d = H_c.d / dt;
+        Calv jc = calving_g(H_c.v, G(i, j), p);
+        d -= jc.d_H * H_c.d;
+        Flux q = flux_x(UL(i, j), HC(i, j - 1), H_c.v, j, nx);
+        d -= (q.d_u * DU(i, j) + q.d_H_l * Hd(i, j - 1).d + q.d_H_r * H_c.d) * dx_inv;
+        q = flux_x(UL(i, j + 1), H_c.v, HC(i, j + 1), j + 1, nx);
+        d += (q.d_u * DU(i, j + 1) + q.d_H_l * H_c.d + q.d_H_r * Hd(i, j + 1).d) * dx_inv;
+        q = flux_y(VL(i, j), HC(i - 1, j), H_c.v, i, ny);
+        d += (q.d_u * DV(i, j) + q.d_H_l * Hd(i - 1, j).d + q.d_H_r * H_c.d) * dx_inv;
+        q = flux_y(VL(i + 1, j), H_c.v, HC(i + 1, j), i + 1, ny);
+        d -= (q.d_u * DV(i + 1, j) + q.d_H_l * H_c.d + q.d_H_r * Hd(i + 1, j).d) * dx_inv;
+        const double m = mask[kc];
+        out[nu + nv + kc] = (1.0 - m) * d + m * H_c.d;
+        return;
+    }
+    // 4c. VJP, cell (i, j): every row's derivative with respect to H(i, j)
+    const double Hc = HC(i, j), eta = E(i, j).v;
+    const double Hl = HC(i, j - 1), Hr = HC(i, j + 1), Ht = HC(i - 1, j), Hb = HC(i + 1, j);
+    const double lc = LM(i, j);
+    Flux qxl = flux_x(UL(i, j), Hl, Hc, j, nx);
+    Flux qxr = flux_x(UL(i, j + 1), Hc, Hr, j + 1, nx);
+    Flux qyt = flux_y(VL(i, j), Ht, Hc, i, ny);
+    Flux qyb = flux_y(VL(i + 1, j), Hc, Hb, i + 1, ny);
+    Calv jc = calving_g(Hc, G(i, j), p);
+    // own mass row (residuals.cu:648-690)
+    double acc = lc / dt;
+    acc += -lc * jc.d_H;
+    acc += -lc * qxl.d_H_r * dx_inv;
+    acc += lc * qxr.d_H_l * dx_inv;
+    acc += lc * qyt.d_H_r * dx_inv;
+    acc += -lc * qyb.d_H_l * dx_inv;
+    acc += dH[kc] * mask[kc];
+    // neighbours' mass rows through the shared faces
+    if (j > 0) acc += LM(i, j - 1) * qxl.d_H_r * dx_inv;
+    if (j < nx - 1) acc += -LM(i, j + 1) * qxr.d_H_l * dx_inv;
+    if (i > 0) acc += -LM(i - 1, j) * qyt.d_H_r * dx_inv;
+    if (i < ny - 1) acc += LM(i + 1, j) * qyb.d_H_l * dx_inv;
+    // momentum rows through eta*H of this cell: normal stresses
+    const double ul = UL(i, j), ur = UL(i, j + 1), vt = VL(i, j), vb = VL(i + 1, j);
+    SigN sx = sigma_xx(ul, ur, vt, vb, 0.0, dx_inv, j, nx);
+    SigN sy = sigma_yy(ul, ur, vt, vb, 0.0, dx_inv, i, ny);
+    const double lu_l = DU(i, j), lu_r = DU(i, j + 1), lv_t = DV(i, j), lv_b = DV(i + 1, j);
+    acc += lu_l * sx.d_eta_H * eta * dx_inv;
+    acc += -lu_r * sx.d_eta_H * eta * dx_inv;
+    acc += -lv_t * sy.d_eta_H * eta * dx_inv;
+    acc += lv_b * sy.d_eta_H * eta * dx_inv;
+    // shear stresses of the four corner vertices
+    const double q4 = 0.25 * eta;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int a = i + (c >> 1), b = j + (c & 1);
+        const double dv_ = EXY(a, b);
+        const double w = DU(a, b) - DU(a - 1, b) - DV(a, b) + DV(a, b - 1);
+        acc += w * dv_ * q4 * dx_inv;
+    }
+    // basal drag (grounding-line derivatives) and driving stress on the four faces
+    {
+        const Ground ga = G(i, j - 1), gb = G(i, j);
+        TauB t = tau_b_g(ul, ga.g, gb.g, ga.gd, gb.gd, BETA(i, j - 1), BETA(i, j), p);
+        TauD d = tau_dx(Hl, Hc, BED(i, j - 1), BED(i, j), dx_inv, j, nx);
+        acc += lu_l * t.d_H_r;
+        acc += -lu_l * d.d_H_r;
+    }
+    {
+        const Ground ga = G(i, j), gb = G(i, j + 1);
+        TauB t = tau_b_g(ur, ga.g, gb.g, ga.gd, gb.gd, BETA(i, j), BETA(i, j + 1), p);
+        TauD d = tau_dx(Hc, Hr, BED(i, j), BED(i, j + 1), dx_inv, j + 1, nx);
+        acc += lu_r * t.d_H_l;
+        acc += -lu_r * d.d_H_l;
+    }
+    {
+        const Ground ga = G(i - 1, j), gb = G(i, j);
+        TauB t = tau_b_g(vt, ga.g, gb.g, ga.gd, gb.gd, BETA(i - 1, j), BETA(i, j), p);
+        TauD d = tau_dy(Ht, Hc, BED(i - 1, j), BED(i, j), dx_inv, i, ny);
+        acc += lv_t * t.d_H_r;
+        acc += -lv_t * d.d_H_r;
+    }
+    {
+        const Ground ga = G(i, j), gb = G(i + 1, j);
+        TauB t = tau_b_g(vb, ga.g, gb.g, ga.gd, gb.gd, BETA(i, j), BETA(i + 1, j), p);
+        TauD d = tau_dy(Hc, Hb, BED(i, j), BED(i + 1, j), dx_inv, i + 1, ny);
+        acc += lv_b * t.d_H_l;
+        acc += -lv_b * d.d_H_l;
+    }
+    emit(nu + nv + kc, acc);
+}
+
 }  // namespace glide

@@ -402,48 +402,52 @@ class IcePhysics:
 
 
 # ---------------------------------------------------------------- losses
-def _nan_zero(d: torch.Tensor) -> torch.Tensor:
+# Misfit and regularisation helpers of the inversion (glide/physics.py:262-411).
+# Unobserved entries (NaN in the observations) give a zero misfit; as in the
+# reference, they still count in the means, and both gradient components are
+# divided by the number of u entries.
+def _misfit(model: torch.Tensor, obs: torch.Tensor) -> torch.Tensor:
+    d = model - obs
     return torch.where(torch.isnan(d), torch.zeros_like(d), d)
 
 
 def huber_loss(u, v, u_obs, v_obs, eps=10.0):
-    """glide/physics.py:262-287"""
-    du, dv = _nan_zero(u - u_obs), _nan_zero(v - v_obs)
-    return torch.sqrt(du ** 2 + eps).mean() + torch.sqrt(dv ** 2 + eps).mean()
+    """Pseudo-Huber misfit: mean sqrt(du^2 + eps) + mean sqrt(dv^2 + eps)
+    (glide/physics.py:262-287)."""
+    return sum(torch.sqrt(_misfit(m, o) ** 2 + eps).mean() for m, o in ((u, u_obs), (v, v_obs)))
 
 
 def huber_grad(u, v, u_obs, v_obs, eps=10.0):
-    """glide/physics.py:290-318"""
-    du, dv = _nan_zero(u - u_obs), _nan_zero(v - v_obs)
+    """d huber_loss / d(u, v) with the reference's normaliser u.size for
+    both components (glide/physics.py:290-318)."""
     n = u.numel()
-    return du / torch.sqrt(du ** 2 + eps) / n, dv / torch.sqrt(dv ** 2 + eps) / n
+    return tuple(d / torch.sqrt(d ** 2 + eps) / n for d in (_misfit(u, u_obs), _misfit(v, v_obs)))
 
 
 def abs_loss(u, v, u_obs, v_obs):
-    """glide/physics.py:320-349"""
-    du = _nan_zero(u.double() - u_obs.double())
-    dv = _nan_zero(v.double() - v_obs.double())
-    return du.abs().mean() + dv.abs().mean()
+    """Mean absolute misfit of u plus that of v, in FP64 (glide/physics.py:320-349)."""
+    return sum(_misfit(m.double(), o.double()).abs().mean() for m, o in ((u, u_obs), (v, v_obs)))
 
 
 def abs_grad(u, v, u_obs, v_obs, eps=10.0):
-    """glide/physics.py:352-379"""
-    du, dv = _nan_zero(u - u_obs), _nan_zero(v - v_obs)
+    """sign(misfit) / u.size for u and v (glide/physics.py:352-379); eps is
+    accepted for the reference's signature and unused."""
     n = u.numel()
-    return torch.sign(du) / n, torch.sign(dv) / n
+    return tuple(torch.sign(d) / n for d in (_misfit(u, u_obs), _misfit(v, v_obs)))
 
 
 def tikhonov_regularization(field: torch.Tensor):
-    """0.5 * (sum dx^2 + sum dy^2) of neighbour differences and its gradient
-    with one-sided ends (glide/physics.py:382-411)."""
-    dxf = field[:, 1:] - field[:, :-1]
-    dyf = field[1:, :] - field[:-1, :]
-    loss = 0.5 * ((dxf ** 2).sum() + (dyf ** 2).sum())
+    """Gradient-smoothness penalty 0.5 (|Dx f|^2 + |Dy f|^2) with Dx, Dy the
+    forward differences between neighbouring cells along rows and columns,
+    and its gradient Dx^T Dx f + Dy^T Dy f (glide/physics.py:382-411):
+    each difference pushes its two cells apart, so the ends of a row or
+    column get one term and interior cells two."""
+    gx = field[:, 1:] - field[:, :-1]
+    gy = field[1:, :] - field[:-1, :]
+    loss = 0.5 * (float((gx * gx).sum()) + float((gy * gy).sum()))
     grad = torch.zeros_like(field)
-    grad[:, 1:-1] -= field[:, 2:] - 2 * field[:, 1:-1] + field[:, :-2]
-    grad[:, 0] -= field[:, 1] - field[:, 0]
-    grad[:, -1] -= field[:, -2] - field[:, -1]
-    grad[1:-1, :] -= field[2:, :] - 2 * field[1:-1, :] + field[:-2, :]
-    grad[0, :] -= field[1, :] - field[0, :]
-    grad[-1, :] -= field[-2, :] - field[-1, :]
-    return float(loss), grad
+    grad[:, 1:] += gx
+    grad[:, :-1] -= gx
+    grad[1:, :] += gy
+    grad[:-1, :] -= gy
+    return loss, grad

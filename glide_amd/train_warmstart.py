@@ -20,6 +20,7 @@ from __future__ import annotations
 import argparse
 import json
 import math
+import os
 import time
 
 import numpy as np
@@ -200,7 +201,9 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--dt", type=float, default=1.0, help="time step of the solved forward steps (a)")
-    ap.add_argument("--dh-scale", type=float, default=1.0, help="network units of the thickness change per m")
+    ap.add_argument("--dh-scale", type=float, nargs="+", default=[0.05],
+                    help="network units of the thickness change per m; several values train one network each "
+                         "on the same solves (written as <out>.dh<scale>.npz)")
     ap.add_argument("--n-test", type=int, default=4)
     ap.add_argument("--test-size", type=int, nargs="*", default=[256, 512, 1024])
     ap.add_argument("--out", default="warmstart_unet.npz")
@@ -211,28 +214,37 @@ def main():
         print(msg, flush=True)
 
     log(f"== generating {args.n_train} solves at {args.size}^2")
-    X, Y, gen = generate(args.n_train, args.size, tol=args.tol, dt=args.dt, log=log, hscale=args.dh_scale)
+    X, Y, gen = generate(args.n_train, args.size, tol=args.tol, dt=args.dt, log=log, hscale=1.0)
     log(f"   {gen}")
-    net = UNet(seed=0)
-    log(f"== training ({args.epochs} epochs, batch {args.batch})")
-    tr = train(net, X, Y, args.size, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
-    del X, Y
-    torch.cuda.empty_cache()
-    net.save(args.out, vel_scale=VEL_SCALE, dh_scale=args.dh_scale, dt=args.dt, train_size=args.size,
-             n_train=args.n_train, epochs=args.epochs, final_val_loss=tr["history"][-1]["val_loss"])
-    net.meta = {"vel_scale": VEL_SCALE, "dh_scale": args.dh_scale}
-    report = {"generate": gen, "train": tr}
-    for size in args.test_size:
-        levels = 6 if size == 1024 else (5 if size == 512 else None)
-        dxs = [1000.0] if size == 1024 else ([1500.0] if size == 512 else [None])
-        for dx in dxs:
+    report = {"generate": gen}
+    for hs in args.dh_scale:
+        tag = "" if len(args.dh_scale) == 1 else f".dh{hs:g}"
+        Yh = Y.clone()
+        Yh.view(Yh.shape[0], -1, N_OUT)[:, :, 2] *= hs     # targets were made with 1 unit per m
+        net = UNet(seed=0)
+        log(f"== training, thickness-change scale {hs:g} ({args.epochs} epochs, batch {args.batch})")
+        tr = train(net, X, Yh, args.size, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log)
+        del Yh
+        torch.cuda.empty_cache()
+        out = args.out[:-4] + tag + ".npz" if args.out.endswith(".npz") else args.out + tag
+        net.save(out, vel_scale=VEL_SCALE, dh_scale=hs, dt=args.dt, train_size=args.size, n_train=args.n_train,
+                 epochs=args.epochs, final_val_loss=tr["history"][-1]["val_loss"])
+        net.meta = {"vel_scale": VEL_SCALE, "dh_scale": hs}
+        rep_h = {"train": tr, "weights": os.path.basename(out)}
+        for size in args.test_size:
+            levels = 6 if size == 1024 else (5 if size == 512 else None)
+            dx = 1000.0 if size == 1024 else (1500.0 if size == 512 else None)
             log(f"== evaluating at {size}^2")
             rows = evaluate(net, range(TEST_SEED0, TEST_SEED0 + args.n_test), size, dx=dx, n_levels=levels,
                             tol=args.tol, dt=args.dt)
             for r in rows:
                 log(f"   {r}")
-            report[f"eval_{size}"] = {"rows": rows, "summary": summarize(rows)}
-            log(f"   summary {report[f'eval_{size}']['summary']}")
+            rep_h[f"eval_{size}"] = {"rows": rows, "summary": summarize(rows)}
+            log(f"   summary {rep_h[f'eval_{size}']['summary']}")
+        report[f"dh_scale_{hs:g}"] = rep_h
+        net.release()
+    del X, Y
+    torch.cuda.empty_cache()
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f, indent=1)

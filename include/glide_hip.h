@@ -330,11 +330,19 @@ int glide_batch_time_kernel(GlideBatch *batch, int kernel, int phase, int ty, in
 
 /* Workgroups per CU of the persistent single-level launches: kernel 0 =
  * the Vanka sweep (default 64, i.e. one tile per workgroup), 1 = the
- * residual (default 4).  Each launch has
+ * residual (default 8).  Each launch has
  * min(tiles, CUs x wg_per_cu) workgroups looping over its tiles.  A launch
  * shape only: results do not depend on it.  Process-wide (atomic); no
  * reference counterpart. */
 int glide_set_persistence(int kernel, int wg_per_cu);
+
+/* Tile heights of the single-level stencils (tiles of 64 x ty points):
+ * kernel 0 = the residual (4, 8 or 16; default 0 = by level size: 16 from
+ * 2^16 to 2^18 cells, 8 above, 4 below), 1 = the JVP and VJP
+ * (4 or 8, default 8; 0 = the per-point form without LDS staging).  A launch
+ * shape only: results are bitwise the same.  Process-wide (atomic); no
+ * reference counterpart. */
+int glide_set_tile_height(int kernel, int ty);
 
 /* Levels with at most max_cells cells run their Vanka sweeps row-split (four
  * waves per 64 cells, each assembling part of the local block; shorter

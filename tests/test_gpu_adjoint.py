@@ -100,6 +100,54 @@ def test_vjp_matches_oracle_and_dot_product(gl, shape):
         assert abs(a - b) <= 1e-12 * (np.abs(lam) @ np.abs(j)), (a, b)
 
 
+@pytest.fixture
+def lin_tiles():
+    """Set the JVP / VJP tile height for one test; restores the default (8)."""
+    yield lambda ty: _lib.call("glide_set_tile_height", 1, ty)
+    _lib.call("glide_set_tile_height", 1, 8)
+
+
+@pytest.mark.parametrize("gl", [False, True])
+@pytest.mark.parametrize("shape", [(7, 9), (33, 70), (64, 129), (130, 64)])
+def test_lin_tile_heights_same_bits(lin_tiles, gl, shape):
+    """The LDS-staged JVP and VJP (tile heights 4 and 8) against the per-point
+    form (0): the JVP equals the oracle bit for bit at every height, the VJP
+    (plain and as the adjoint residual base - J^T lambda) the per-point VJP."""
+    ny, nx = shape
+    s = random_state(ny=ny, nx=nx, seed=3 * ny + nx)
+    s["mask"][ny // 3, nx // 2] = 1.0
+    prm = O.params(gl_derivatives=gl)
+    rng = np.random.default_rng(ny * nx)
+    d, lam, base = (rng.normal(size=s["U"].shape) for _ in range(3))
+    ref = O.jvp(ny, nx, s["dx"], s["dt"], prm, s["U"], d, s["bed"], s["B"], s["beta"], s["mask"], s["gamma"])
+    g = _lib.GlideGeom(ny, nx, s["dx"], s["dt"])
+    lev = (_lib.GlideLevel * 1)()
+    lev[0].g = g
+    lev[0].p = cparams(prm)
+    Ut, lt, bt = T(s["U"]), T(lam), T(base)
+    cells = {k: T(s[k]) for k in ("bed", "B", "beta", "mask", "gamma")}
+    for k in ("bed", "B", "beta", "mask", "gamma"):
+        setattr(lev[0], k, cells[k].data_ptr())
+    lev[0].U, lev[0].Lambda, lev[0].f_adj = Ut.data_ptr(), lt.data_ptr(), bt.data_ptr()
+    outs = {}
+    for ty in (0, 4, 8):
+        lin_tiles(ty)
+        jo = torch.zeros(ref.size, dtype=torch.float64, device=DEV)
+        vo = torch.zeros(ref.size, dtype=torch.float64, device=DEV)
+        ra = torch.zeros(ref.size, dtype=torch.float64, device=DEV)
+        lev[0].r_adj = ra.data_ptr()
+        _lib.call("glide_jvp", ctypes.byref(g), ctypes.byref(cparams(prm)), P(jo), P(Ut), P(T(d)), *_args(s), stream())
+        _lib.call("glide_vjp", ctypes.byref(g), ctypes.byref(cparams(prm)), P(vo), P(Ut), P(lt), *_args(s), stream())
+        _lib.call("glide_adjoint_residual", ctypes.byref(lev[0]), stream())
+        torch.cuda.synchronize()
+        outs[ty] = (jo.cpu().numpy(), vo.cpu().numpy(), ra.cpu().numpy())
+        assert np.array_equal(outs[ty][0], ref), (ty, np.abs(outs[ty][0] - ref).max())
+    for ty in (4, 8):
+        assert np.array_equal(outs[ty][1], outs[0][1]), ty
+        assert np.array_equal(outs[ty][2], outs[0][2]), ty
+    assert np.array_equal(outs[0][2], base - outs[0][1])
+
+
 @pytest.mark.parametrize("color", [0, 1])
 def test_adjoint_pass_bitwise(color):
     ny, nx = 20, 37

@@ -90,6 +90,29 @@ def test_residual_bitwise(shape, gl):
     assert np.array_equal(r.cpu().numpy(), refF - sub)
 
 
+@pytest.mark.parametrize("ty", [4, 8, 16])
+@pytest.mark.parametrize("shape", [(8, 10), (37, 45), (64, 129), (33, 128), (300, 260)])
+def test_residual_tile_heights_bitwise(ty, shape):
+    """The residual on 64 x 4, 64 x 8 and 64 x 16 tiles equals the oracle bit for bit."""
+    ny, nx = shape
+    s = random_state(ny=ny, nx=nx, seed=2 * ny + nx)
+    s["mask"][ny // 2, nx // 3] = 1.0
+    prm = O.params(gl_derivatives=True)
+    ref = O.residual(ny, nx, s["dx"], s["dt"], prm, s["U"], s["f"], s["bed"], s["B"], s["beta"], s["mask"], s["gamma"])
+    g = _lib.GlideGeom(ny, nx, s["dx"], s["dt"])
+    r = torch.zeros(ref.size, dtype=torch.float64, device=DEV)
+    U, f, bed, B, beta, mask, gamma = (T(s[k]) for k in ("U", "f", "bed", "B", "beta", "mask", "gamma"))
+    _lib.call("glide_set_tile_height", 0, ty)
+    try:
+        _lib.call("glide_residual", ctypes.byref(g), ctypes.byref(cparams(prm)), P(r), P(U), P(f), None, P(bed), P(B),
+                  P(beta), P(mask), P(gamma), stream())
+        torch.cuda.synchronize()
+    finally:
+        _lib.call("glide_set_tile_height", 0, 0)
+    out = r.cpu().numpy()
+    assert np.array_equal(out, ref), np.abs(out - ref).max()
+
+
 @pytest.fixture
 def row_split():
     """Set the row-split cell limit for one test (0: every level sweeps with
