@@ -1290,8 +1290,11 @@ int launch_vjp(const GlideGeom &g, const Params &p, double *out, const double *b
 {
     const int ty = g_lin_ty;
     const dim3 grid4(cdiv(g.nx + 1, TX), cdiv(g.ny + 1, 4)), grid8(cdiv(g.nx + 1, TX), cdiv(g.ny + 1, 8));
+    const dim3 grid6(cdiv(g.nx + 1, TX), cdiv(g.ny + 1, 6));
     if (ty == 8)
         hipLaunchKernelGGL(k_vjp<8>, grid8, dim3(TX, 8), 0, st, g, p, out, base, U, lam, bed, B, beta, mask);
+    else if (ty == 6)
+        hipLaunchKernelGGL(k_vjp<6>, grid6, dim3(TX, 6), 0, st, g, p, out, base, U, lam, bed, B, beta, mask);
     else if (ty == 4)
         hipLaunchKernelGGL(k_vjp<4>, grid4, dim3(TX, 4), 0, st, g, p, out, base, U, lam, bed, B, beta, mask);
     else
@@ -1468,7 +1471,7 @@ int glide_set_tile_height(int kernel, int ty)
             return fail(GLIDE_E_ARG, "residual tile height must be 0 (by level size), 4, 8 or 16");
         g_res_ty = ty;
     } else if (kernel == 1) {
-        if (ty != 0 && ty != 4 && ty != 8) return fail(GLIDE_E_ARG, "JVP/VJP tile height must be 0, 4 or 8");
+        if (ty != 0 && ty != 4 && ty != 6 && ty != 8) return fail(GLIDE_E_ARG, "JVP/VJP tile height must be 0, 4, 6 or 8");
         g_lin_ty = ty;
     } else {
         return fail(GLIDE_E_ARG, "kernel must be 0 (residual) or 1 (JVP/VJP)");
@@ -1551,8 +1554,11 @@ int glide_jvp(const GlideGeom *g, const GlideParams *p, double *j, const double 
     (void)gamma;
     const int ty = g_lin_ty;
     const dim3 grid4(cdiv(g->nx + 1, TX), cdiv(g->ny + 1, 4)), grid8(cdiv(g->nx + 1, TX), cdiv(g->ny + 1, 8));
+    const dim3 grid6(cdiv(g->nx + 1, TX), cdiv(g->ny + 1, 6));
     if (ty == 8)
         hipLaunchKernelGGL(k_jvp<8>, grid8, dim3(TX, 8), 0, S(stream), *g, to_params(*p), j, U, dU, bed, B, beta, mask);
+    else if (ty == 6)
+        hipLaunchKernelGGL(k_jvp<6>, grid6, dim3(TX, 6), 0, S(stream), *g, to_params(*p), j, U, dU, bed, B, beta, mask);
     else if (ty == 4)
         hipLaunchKernelGGL(k_jvp<4>, grid4, dim3(TX, 4), 0, S(stream), *g, to_params(*p), j, U, dU, bed, B, beta, mask);
     else

@@ -493,7 +493,6 @@ struct LinLDS {
     double g[TY + 2][TX + 2], gd[TY + 2][TX + 2];    // grounding sigmoid and its derivative
     double txx[TY + 2][TX + 2], tyy[TY + 2][TX + 2]; // directional derivatives of sigma_xx, sigma_yy
     double txy[TY + 1][TX + 1];                      // ... of sigma_xy, vertices (i0+a, j0+b)
-    double exy[TY + 1][TX + 1];                      // 2 eps_xy at the vertices (0 on the boundary); VJP only
 };
 
 // eta*H as a Dual: cell and vertex average (viscosity.cu:150-217)
@@ -595,13 +594,15 @@ __device__ __forceinline__ void lin_tile_lds(const GlideGeom &g, const Params &p
         const Dual ut = Ud(vi - 1, vj), ub = Ud(vi, vj), vl = Vd(vi, vj - 1), vr = Vd(vi, vj);
         const SigS s = sigma_xy(ut.v, ub.v, vl.v, vr.v, eH.v, dx_inv, vi, vj, ny, nx);
         S.txy[a][b] = sigS_jvp(s, ut.d, ub.d, vl.d, vr.d, eH.d);
-        if (VJP) S.exy[a][b] = s.d_eta_H;
     }
     __syncthreads();
     auto TXX = [&](int ci, int cj) { return S.txx[ci - i0 + 1][cj - j0 + 1]; };
     auto TYY = [&](int ci, int cj) { return S.tyy[ci - i0 + 1][cj - j0 + 1]; };
     auto TXY = [&](int vi, int vj) { return S.txy[vi - i0][vj - j0]; };
-    auto EXY = [&](int vi, int vj) { return S.exy[vi - i0][vj - j0]; };
+    // 2 eps_xy at vertex (vi, vj), 0 on boundary vertices
+    auto EXY = [&](int vi, int vj) {
+        return sigma_xy(UL(vi - 1, vj), UL(vi, vj), VL(vi, vj - 1), VL(vi, vj), 0.0, dx_inv, vi, vj, ny, nx).d_eta_H;
+    };
     const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
     if (i > ny || j > nx) return;
     auto emit = [&](size_t k, double val) { out[k] = (VJP && base) ? base[k] - val : val; };

@@ -100,9 +100,9 @@ def max_levels(ny: int, nx: int, coarsest: int = 16) -> int:
 def ice_slab(ny: int, nx: int, dx: float, seed: int, smb: float = 0.3) -> dict:
     """Ice-covered, grounded slab (no margin): H = 800 + 200 sin cos m on a
     bed 300 m + 0.004 x + a 20 m GRF, log10 beta ~ 3 + 0.3 * (unit GRF at
-    8 km).  Without an ice margin the FASCD iteration converges to machine
-    precision, which the gradient checks and the time-to-tolerance
-    measurement need."""
+    8 km).  On these slabs FASCD converges to about 1e-13 relative residual
+    with the reference's restriction too, which the finite-difference
+    gradient checks need."""
     rng = np.random.default_rng(seed)
     y = np.arange(ny)[:, None]
     x = np.arange(nx)[None, :]
@@ -140,8 +140,10 @@ def ice_sheet(ny: int, nx: int, dx: float, seed: int, smb: float = 0.3) -> dict:
     and test geometries): H = 700 + 250 * (unit GRF at 30 km), clipped to
     [250, 1500] m, on a bed 300 m + 0.003 x + 80 * (unit GRF at 20 km)
     (grounded everywhere: bed > 0 > -0.917 H), log10 beta ~ 3 + 0.4 * (unit
-    GRF at 8 km), uniform B and smb.  With no ice margin FASCD converges to
-    machine precision, so forward steps can be solved to 1e-8."""
+    GRF at 8 km), uniform B and smb.  No ice margin and no active thickness
+    constraint; with the restricted residual's boundary rows weighted 0.5
+    every measured sheet's forward step reaches 1e-8 (with the reference's
+    weight 1 about a quarter stall above it, at the domain edge)."""
     rng = np.random.default_rng(seed)
     x = (np.arange(nx)[None, :] + 0.5) * dx
     H = np.clip(700.0 + 250.0 * gaussian_field(ny, nx, dx, 30e3, rng), 250.0, 1500.0)

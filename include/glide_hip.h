@@ -339,7 +339,7 @@ int glide_set_persistence(int kernel, int wg_per_cu);
 /* Tile heights of the single-level stencils (tiles of 64 x ty points):
  * kernel 0 = the residual (4, 8 or 16; default 0 = by level size: 16 from
  * 2^16 to 2^18 cells, 8 above, 4 below), 1 = the JVP and VJP
- * (4 or 8, default 8; 0 = the per-point form without LDS staging).  A launch
+ * (4, 6 or 8, default 8; 0 = the per-point form without LDS staging).  A launch
  * shape only: results are bitwise the same.  Process-wide (atomic); no
  * reference counterpart. */
 int glide_set_tile_height(int kernel, int ty);

@@ -130,7 +130,7 @@ def test_lin_tile_heights_same_bits(lin_tiles, gl, shape):
         setattr(lev[0], k, cells[k].data_ptr())
     lev[0].U, lev[0].Lambda, lev[0].f_adj = Ut.data_ptr(), lt.data_ptr(), bt.data_ptr()
     outs = {}
-    for ty in (0, 4, 8):
+    for ty in (0, 4, 6, 8):
         lin_tiles(ty)
         jo = torch.zeros(ref.size, dtype=torch.float64, device=DEV)
         vo = torch.zeros(ref.size, dtype=torch.float64, device=DEV)
@@ -142,7 +142,7 @@ def test_lin_tile_heights_same_bits(lin_tiles, gl, shape):
         torch.cuda.synchronize()
         outs[ty] = (jo.cpu().numpy(), vo.cpu().numpy(), ra.cpu().numpy())
         assert np.array_equal(outs[ty][0], ref), (ty, np.abs(outs[ty][0] - ref).max())
-    for ty in (4, 8):
+    for ty in (4, 6, 8):
         assert np.array_equal(outs[ty][1], outs[0][1]), ty
         assert np.array_equal(outs[ty][2], outs[0][2]), ty
     assert np.array_equal(outs[0][2], base - outs[0][1])
