@@ -16,8 +16,11 @@ Also reported on the same JSON line:
   - learned_batched: 16 such sheets as one ensemble, plain vs learned;
   - reference_schedule: the parity workload cap_512 (floating margin) on the
     reference's fixed 3-V-cycle step;
-  - roofline: the finest-level Vanka sweep (FP64 VALU), HIP-event timed on
-    the solver's stream, priced with the frozen per-cell flop count;
+  - roofline: the finest-level Vanka sweep (FP64 VALU), the largest kernel
+    time share of the headline, HIP-event timed on the solver's stream and
+    priced with the frozen per-cell flop count; small_level_sweep_roofline:
+    the same for the coarsest level, where one wave's Newton chain bounds a
+    sweep;
   - stencil_roofline / ensemble: the residual stencil (HBM), single domain
     and over the 64-member ensemble, whose batched step is timed against
     solving the members one by one;
@@ -68,10 +71,14 @@ def parse():
 # WRITE_SIZE (one run each) and calibrated in the same program on kernels of
 # known traffic: a 256 MiB torch.sum reads 131,134 KiB of FETCH_SIZE (bytes =
 # 2048 x FETCH_SIZE) and a 256 MiB fill writes 262,144 KiB of WRITE_SIZE
-# (bytes = 1024 x WRITE_SIZE).  Vanka sweep + apply pass: FETCH 13,204.2 + 8,708.5,
-# WRITE 10,240 + 6,248; residual (cell tiles): FETCH 14,133.8, WRITE 6,248.
-TRAFFIC_512 = {"vanka": (2 * (13204.2 + 8708.5) + 10240.0 + 6248.0) * 1024,
-               "residual": (2 * 14133.8 + 6248.0) * 1024}
+# (bytes = 1024 x WRITE_SIZE).  Vanka sweep + apply pass: FETCH 13,209.6 + 8,710.5,
+# WRITE 10,240 + 6,248; residual (cell tiles, 64 x 16): FETCH 14,137.4, WRITE
+# 6,248; JVP / VJP (64 x 4 tiles, XCD-chunked order): FETCH 11,717.3 / 11,872.0,
+# WRITE 6,179.7 / 6,182.1.
+TRAFFIC_512 = {"vanka": (2 * (13209.6 + 8710.5) + 10240.0 + 6248.0) * 1024,
+               "residual": (2 * 14137.4 + 6248.0) * 1024,
+               "jvp": (2 * 11717.3 + 6179.7) * 1024,
+               "vjp": (2 * 11872.0 + 6182.1) * 1024}
 
 
 def kernel_rooflines(ph, reps=20):
@@ -142,7 +149,8 @@ def kernel_rooflines(ph, reps=20):
                 "work": f"{nbytes} algorithmic bytes"}
 
     return {
-        "finest_sweep_roofline": {"kernel": "k_vanka<false> + k_vanka_apply (finest full sweep, 512^2)", "bound": "valu",
+        "finest_sweep_roofline": {"kernel": "k_vanka<false, 2> + k_vanka_apply<false> (finest full sweep, 512^2)",
+                     "bound": "valu",
                      "note": "FP64 vector arithmetic (no MFMA); peak is MI355X FP64 78.6 TFLOP/s",
                      "achieved": round(vanka_tf, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(vanka_tf / PEAK_FP64_TFLOPS, 4),
@@ -151,8 +159,10 @@ def kernel_rooflines(ph, reps=20):
                      "work": f"{nc} cells x (eta + 10 Newton x {FLOPS_NEWTON_ITER} flop)"},
         "stencil_roofline": hbm("k_residual (512^2)", out["residual"], res_bytes,
                                 TRAFFIC_512["residual"] if (g.ny, g.nx) == (512, 512) else None),
-        "jvp_roofline": hbm(f"k_jvp ({g.ny}x{g.nx})", out["jvp"], lin_bytes),
-        "vjp_roofline": hbm(f"k_vjp ({g.ny}x{g.nx})", out["vjp"], lin_bytes),
+        "jvp_roofline": hbm(f"k_jvp<4> ({g.ny}x{g.nx})", out["jvp"], lin_bytes,
+                            TRAFFIC_512.get("jvp") if (g.ny, g.nx) == (512, 512) else None),
+        "vjp_roofline": hbm(f"k_vjp<4> ({g.ny}x{g.nx})", out["vjp"], lin_bytes,
+                            TRAFFIC_512.get("vjp") if (g.ny, g.nx) == (512, 512) else None),
     }
 
 
@@ -459,7 +469,7 @@ def level_sweep_roofline(ph, level: int, n_sweeps: int, reps: int = 3):
     ms = e0.elapsed_time(e1) / (reps * n_sweeps)
     nc = g.ny * g.nx
     tf = sweep_flops(nc) / (ms * 1e-3) / 1e12
-    return {"kernel": f"k_vanka_rows<false> + k_vanka_apply<false> (full sweep of the {g.ny}x{g.nx} level)",
+    return {"kernel": f"k_vanka<false, 1> + k_vanka_apply<false> (full sweep of the {g.ny}x{g.nx} level)",
             "bound": "valu", "note": "FP64 vector arithmetic (no MFMA); peak = MI355X FP64 78.6 TFLOP/s; one "
             "lane per cell, latency of the per-cell Newton chain plus two launches per sweep",
             "achieved": round(tf, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
@@ -705,8 +715,8 @@ def main():
     plain_ms = float(np.mean([r["plain_ms"] for r in both])) if both else None
     learned_ms = float(np.mean([r["learned_ms"] for r in both])) if (learned and both) else None
     rl = kernel_rooflines(ph)
-    # the coarsest level's 200-sweep solve: the largest kernel-time share of a headline V-cycle
-    coarse = level_sweep_roofline(ph, LEVELS512 - 1, 200)
+    finest = rl.pop("finest_sweep_roofline")
+    coarse = level_sweep_roofline(ph, LEVELS512 - 1, 50)
     del ph
     torch.cuda.empty_cache()
     value = learned_ms if learned else plain_ms
@@ -721,8 +731,9 @@ def main():
                                + (" with the learned warm start" if learned else " (no trained weights: plain)")
                                + f"; FASCD with the tolerance schedule {TS} (boundary rows of restricted "
                                "residuals weighted 0.5: with the reference's weight 1 FASCD stalls above 1e-8 on "
-                               "sheets with normal stress at the domain edge; 50 coarsest-level sweeps instead of "
-                               "200: same V-cycle counts; see reference_settings); mean over the steps that reach "
+                               "sheets with normal stress at the domain edge; 5 coarsest-level sweeps instead of "
+                               "200: same V-cycle counts, the finest levels' smoothing sets them; see "
+                               "reference_settings); mean over the steps that reach "
                                "1e-8 under both starts.  Not the 512^2 parity workload cap_512: its floating margin "
                                "stalls FASCD near 1e-3, so it has no time to 1e-8 (its 3-V-cycle step: "
                                "reference_schedule)",
@@ -735,7 +746,8 @@ def main():
         "vcycles_plain": [r["vcycles_plain"] for r in rows],
         "vcycles_learned": [r.get("vcycles_learned") for r in rows],
         "fallbacks": int(sum(bool(r.get("fallback")) for r in rows)),
-        "roofline": coarse,
+        "roofline": finest,
+        "small_level_sweep_roofline": coarse,
         **rl,
     }
     if args.headline_only:

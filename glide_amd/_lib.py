@@ -135,7 +135,7 @@ SIGNATURES = {
     "glide_unet_apply_prediction": (c_int, [c_void_p, c_int, c_int, c_double, c_double, DP, DP, DP, DP, DP,
                                             c_void_p]),
     "glide_set_persistence": (c_int, [c_int, c_int]),
-    "glide_set_row_split": (c_int, [ctypes.c_long]),
+    "glide_set_small_levels": (c_int, [ctypes.c_long, c_int]),
     "glide_set_tile_height": (c_int, [c_int, c_int]),
     "glide_count_nonfinite": (c_int, [DP, ctypes.c_long, c_void_p, POINTER(ctypes.c_ulonglong), c_void_p]),
     "glide_vti_block_bytes": (ctypes.c_size_t, [POINTER(GlideVtiArray), c_int, c_int, c_int]),

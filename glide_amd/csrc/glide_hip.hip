@@ -64,14 +64,15 @@ inline hipStream_t S(void *s) { return (hipStream_t)s; }
 // not depend on them.
 std::atomic<int> g_wg_per_cu[2] = {{64}, {8}};
 // Levels with at most this many cells use the row-split Vanka sweep.
-std::atomic<long> g_row_split_max_cells{128 * 128};
+std::atomic<long> g_small_max_cells{128 * 128};
+std::atomic<int> g_small_variant{0};
 // Tile heights (64 x TY points) of the single-level residual (4, 8, 16; 0 =
 // by level size) and of the JVP / VJP (0 = the per-point form, 4, 8); set by
 // glide_set_tile_height.  Launch shapes only: results are the same bits.
 // Measured at 512^2: residual 64 x 16 tiles 11.3 us, 64 x 8 12.0, 64 x 4 13.4;
 // at 1024^2: 64 x 8 39.7 us, 64 x 16 42.0, 64 x 4 46.4 (8 workgroups per CU).
 std::atomic<int> g_res_ty{0};
-std::atomic<int> g_lin_ty{8};
+std::atomic<int> g_lin_ty{4};
 int residual_ty(int ny, int nx)
 {
     const int ty = g_res_ty;
@@ -133,10 +134,12 @@ __global__ __launch_bounds__(TX *RTY) void k_residual(GlideGeom g, Params p, dou
 // cd (5 x nc, structure of arrays) receives 0.5*(x - x0) for the four faces
 // and (H - H0) for the cell.
 // ---------------------------------------------------------------------------
-// two waves per SIMD: the register budget both instantiated cell bodies
-// (interior and general) must share
-template <bool LOCAL>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_vanka(GlideGeom g, Params p,
+// WPE waves per SIMD: the register budget both instantiated cell bodies
+// (interior and general) must share.  Large levels take 2 (occupancy hides
+// the Newton chain's latency); small levels, whose few waves each run alone
+// on a SIMD, take 1 (the whole register file: no spills, longer schedules).
+template <bool LOCAL, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_vanka(GlideGeom g, Params p,
                                                                                    const double *__restrict__ U,
                                               const double *__restrict__ f, const double *__restrict__ bed,
                                               const double *__restrict__ B, const double *__restrict__ beta,
@@ -154,12 +157,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 
 // ---------------------------------------------------------------------------
 // Row-split Vanka sweep for small levels, where a sweep takes as long as one
-// wave's Newton chain.  A workgroup holds 4 waves x 64 cells of one grid row,
-// one wave per SIMD; wave w assembles rows {0}, {1}, {2}, {3, 4} of each
-// cell's 5x5 block (vanka_assemble with a row mask: each entry receives the
-// operations it receives in the full assembly), the rows meet in LDS (double
-// buffered, one barrier per Newton iteration) and every wave solves the
-// block and updates its copy of the unknowns.  Same bits as k_vanka.
+// wave's Newton chain.  A workgroup holds 5 waves x 64 cells of one grid row;
+// wave w assembles row w of each cell's 5x5 block (vanka_assemble with a row
+// mask: each entry receives the operations it receives in the full
+// assembly), the rows meet in LDS (double buffered, one barrier per Newton
+// iteration) and every wave solves the block and updates its copy of the
+// unknowns.  Same bits as k_vanka.
 // ---------------------------------------------------------------------------
 template <int ROWS>
 __device__ __forceinline__ void rows_to_lds(const CellNbhd &N, const Params &p, int i, int j, int ny, int nx,
@@ -178,8 +181,10 @@ __device__ __forceinline__ void rows_to_lds(const CellNbhd &N, const Params &p, 
     }
 }
 
+constexpr int ROW_WAVES = 5;
+
 template <bool LOCAL>
-__global__ __launch_bounds__(256) void k_vanka_rows(GlideGeom g, Params p, const double *__restrict__ U,
+__global__ __launch_bounds__(64 * ROW_WAVES) void k_vanka_rows(GlideGeom g, Params p, const double *__restrict__ U,
                                                     const double *__restrict__ f, const double *__restrict__ bed,
                                                     const double *__restrict__ B, const double *__restrict__ beta,
                                                     const double *__restrict__ gamma,
@@ -198,7 +203,7 @@ __global__ __launch_bounds__(256) void k_vanka_rows(GlideGeom g, Params p, const
         mine = mine && blk[(i / RB) * nbx + j / RB];
         if (!__syncthreads_or(mine)) return;   // uniform: the whole tile is idle
     }
-    for (int k = w * 64 + lane; k < 3 * 66; k += 256) {   // viscosity of cells [i-1, i+1] x [j0-1, j0+64]
+    for (int k = w * 64 + lane; k < 3 * 66; k += 64 * ROW_WAVES) {   // viscosity of cells [i-1, i+1] x [j0-1, j0+64]
         const int a = k / 66, b = k - a * 66;
         const int ci = i - 1 + a, cj = j0 - 1 + b;
         eta_s[a][b] = (ci >= 0 && ci < ny && cj >= 0 && cj < nx) ? eta_cell(ci, cj, u, v, B, p, g.dx, ny, nx) : 0.0;
@@ -223,8 +228,10 @@ __global__ __launch_bounds__(256) void k_vanka_rows(GlideGeom g, Params p, const
             rows_to_lds<2>(N, p, i, jj, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, x, lane);
         else if (w == 2)
             rows_to_lds<4>(N, p, i, jj, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, x, lane);
+        else if (w == 3)
+            rows_to_lds<8>(N, p, i, jj, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, x, lane);
         else
-            rows_to_lds<24>(N, p, i, jj, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, x, lane);
+            rows_to_lds<16>(N, p, i, jj, ny, nx, dx_inv, dt, u_l, u_r, v_t, v_b, H_c, th, x, lane);
         // one barrier per iteration: buffer k & 1 is written again at k + 2,
         // after every wave has passed barrier k + 1 and so finished reading it
         __syncthreads();
@@ -837,7 +844,10 @@ __global__ __launch_bounds__(NT) void k_vjp_point(GlideGeom g, Params p, double 
     }
 }
 
-// LDS-staged JVP and VJP on 64 x LTY point tiles (lin_tile_lds)
+// LDS-staged JVP and VJP on 64 x LTY point tiles (lin_tile_lds).  Tiles are
+// numbered down the columns of tiles and dealt to the XCDs in chunks of
+// consecutive tiles (chunked_tile), so a chunk's vertical neighbours, which
+// share the larger halos, meet in one L2.
 template <int LTY>
 __global__ __launch_bounds__(TX *LTY) void k_jvp(GlideGeom g, Params p, double *__restrict__ jo,
                                                  const double *__restrict__ U, const double *__restrict__ dU,
@@ -845,7 +855,10 @@ __global__ __launch_bounds__(TX *LTY) void k_jvp(GlideGeom g, Params p, double *
                                                  const double *__restrict__ beta, const double *__restrict__ mask)
 {
     __shared__ LinLDS<TX, LTY> S;
-    lin_tile_lds<TX, LTY, false>(g, p, jo, nullptr, U, dU, bed, B, beta, mask, blockIdx.x, blockIdx.y, S);
+    const int tiles_y = cdiv(g.ny + 1, LTY), total = cdiv(g.nx + 1, TX) * tiles_y;
+    const int t = chunked_tile(blockIdx.x, total);
+    if (t >= total) return;
+    lin_tile_lds<TX, LTY, false>(g, p, jo, nullptr, U, dU, bed, B, beta, mask, t / tiles_y, t % tiles_y, S);
 }
 template <int LTY>
 __global__ __launch_bounds__(TX *LTY) void k_vjp(GlideGeom g, Params p, double *__restrict__ out,
@@ -855,8 +868,13 @@ __global__ __launch_bounds__(TX *LTY) void k_vjp(GlideGeom g, Params p, double *
                                                  const double *__restrict__ mask)
 {
     __shared__ LinLDS<TX, LTY> S;
-    lin_tile_lds<TX, LTY, true>(g, p, out, base, U, lam, bed, B, beta, mask, blockIdx.x, blockIdx.y, S);
+    const int tiles_y = cdiv(g.ny + 1, LTY), total = cdiv(g.nx + 1, TX) * tiles_y;
+    const int t = chunked_tile(blockIdx.x, total);
+    if (t >= total) return;
+    lin_tile_lds<TX, LTY, true>(g, p, out, base, U, lam, bed, B, beta, mask, t / tiles_y, t % tiles_y, S);
 }
+// grid of the staged JVP / VJP
+inline dim3 lin_grid(const GlideGeom &g, int ty) { return dim3(chunked_grid(cdiv(g.nx + 1, TX) * cdiv(g.ny + 1, ty))); }
 
 // Adjoint red-black Vanka pass, in place: cells of one colour own disjoint
 // dofs, so lam is read and written by one lane per dof
@@ -1051,8 +1069,9 @@ int launch_sweeps(const GlideGeom &g, const Params &p, double *U, const double *
                   double *scratch, hipStream_t st)
 {
     const int nbx = nbx_of(g.nx);
-    const bool rows = (long)g.ny * g.nx <= g_row_split_max_cells;
-    const dim3 rgrid(cdiv(g.nx, 64), g.ny), rblock(64, 4);
+    const bool small = (long)g.ny * g.nx <= g_small_max_cells;
+    const bool rows = small && g_small_variant == 1;
+    const dim3 rgrid(cdiv(g.nx, 64), g.ny), rblock(64, ROW_WAVES);
     for (int s = 0; s < n_sweeps; s++) {
         if (rows) {
             if (blk) {
@@ -1069,13 +1088,21 @@ int launch_sweeps(const GlideGeom &g, const Params &p, double *U, const double *
             continue;
         }
         if (blk) {
-            hipLaunchKernelGGL(k_vanka<true>, vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B, beta,
-                               gamma, blk, nbx, n_newton, scratch);
+            if (small)
+                hipLaunchKernelGGL((k_vanka<true, 1>), vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B,
+                                   beta, gamma, blk, nbx, n_newton, scratch);
+            else
+                hipLaunchKernelGGL((k_vanka<true, 2>), vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B,
+                                   beta, gamma, blk, nbx, n_newton, scratch);
             hipLaunchKernelGGL(k_vanka_apply<true>, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g.ny, g.nx, U,
                                scratch, blk, nbx);
         } else {
-            hipLaunchKernelGGL(k_vanka<false>, vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed, B, beta,
-                               gamma, blk, nbx, n_newton, scratch);
+            if (small)
+                hipLaunchKernelGGL((k_vanka<false, 1>), vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed,
+                                   B, beta, gamma, blk, nbx, n_newton, scratch);
+            else
+                hipLaunchKernelGGL((k_vanka<false, 2>), vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed,
+                                   B, beta, gamma, blk, nbx, n_newton, scratch);
             hipLaunchKernelGGL(k_vanka_apply<false>, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g.ny, g.nx, U,
                                scratch, blk, nbx);
         }
@@ -1289,8 +1316,7 @@ int launch_vjp(const GlideGeom &g, const Params &p, double *out, const double *b
                const double *bed, const double *B, const double *beta, const double *mask, hipStream_t st)
 {
     const int ty = g_lin_ty;
-    const dim3 grid4(cdiv(g.nx + 1, TX), cdiv(g.ny + 1, 4)), grid8(cdiv(g.nx + 1, TX), cdiv(g.ny + 1, 8));
-    const dim3 grid6(cdiv(g.nx + 1, TX), cdiv(g.ny + 1, 6));
+    const dim3 grid4 = lin_grid(g, 4), grid8 = lin_grid(g, 8), grid6 = lin_grid(g, 6);
     if (ty == 8)
         hipLaunchKernelGGL(k_vjp<8>, grid8, dim3(TX, 8), 0, st, g, p, out, base, U, lam, bed, B, beta, mask);
     else if (ty == 6)
@@ -1457,10 +1483,12 @@ int glide_vcycle_work_counts(GlideVcycleWork *w, long *run, long *skipped, int r
     return GLIDE_OK;
 }
 
-int glide_set_row_split(long max_cells)
+int glide_set_small_levels(long max_cells, int variant)
 {
     if (max_cells < 0) return fail(GLIDE_E_ARG, "max_cells must be >= 0");
-    g_row_split_max_cells = max_cells;
+    if (variant != 0 && variant != 1) return fail(GLIDE_E_ARG, "small-level variant must be 0 (per-cell) or 1 (row split)");
+    g_small_max_cells = max_cells;
+    g_small_variant = variant;
     return GLIDE_OK;
 }
 
@@ -1553,8 +1581,7 @@ int glide_jvp(const GlideGeom *g, const GlideParams *p, double *j, const double 
     if (!p || !j || !U || !dU || !bed || !B || !beta || !mask) return fail(GLIDE_E_NULL, "jvp: NULL argument");
     (void)gamma;
     const int ty = g_lin_ty;
-    const dim3 grid4(cdiv(g->nx + 1, TX), cdiv(g->ny + 1, 4)), grid8(cdiv(g->nx + 1, TX), cdiv(g->ny + 1, 8));
-    const dim3 grid6(cdiv(g->nx + 1, TX), cdiv(g->ny + 1, 6));
+    const dim3 grid4 = lin_grid(*g, 4), grid8 = lin_grid(*g, 8), grid6 = lin_grid(*g, 6);
     if (ty == 8)
         hipLaunchKernelGGL(k_jvp<8>, grid8, dim3(TX, 8), 0, S(stream), *g, to_params(*p), j, U, dU, bed, B, beta, mask);
     else if (ty == 6)

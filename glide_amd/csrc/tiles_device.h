@@ -56,6 +56,24 @@ __device__ __forceinline__ void tile_loop(int total, const Body &body)
     }
 }
 
+// One tile per workgroup with the tiles dealt to the XCDs in chunks of up to
+// TCH consecutive tiles, as tile_loop deals them: workgroup b (on XCD b % 8)
+// takes tile (x + 8 c) tch + o for x = b % 8, c = (b / 8) / tch, o = (b / 8)
+// % tch.  Over a grid of a multiple of 8 tch workgroups this is a bijection;
+// workgroups whose tile is past the last one (t >= total) have no tile.
+__device__ __forceinline__ int chunked_tile(int b, int total)
+{
+    const int tch = max(1, min(TCH, total / (2 * NXCD)));
+    const int k = b / NXCD;
+    return (b % NXCD + NXCD * (k / tch)) * tch + k % tch;
+}
+__host__ __device__ inline int chunked_grid(int total)
+{
+    const int tch = total / (2 * NXCD) < 1 ? 1 : (total / (2 * NXCD) > TCH ? TCH : total / (2 * NXCD));
+    const int span = NXCD * tch;
+    return (total + span - 1) / span * span;
+}
+
 // ---------------------------------------------------------------------------
 // viscosity tile: cells [i0-1, i0+TY] x [j0-1, j0+TX] into LDS (0 outside)
 // ---------------------------------------------------------------------------

@@ -339,17 +339,19 @@ int glide_set_persistence(int kernel, int wg_per_cu);
 /* Tile heights of the single-level stencils (tiles of 64 x ty points):
  * kernel 0 = the residual (4, 8 or 16; default 0 = by level size: 16 from
  * 2^16 to 2^18 cells, 8 above, 4 below), 1 = the JVP and VJP
- * (4, 6 or 8, default 8; 0 = the per-point form without LDS staging).  A launch
+ * (4, 6 or 8, default 4; 0 = the per-point form without LDS staging).  A launch
  * shape only: results are bitwise the same.  Process-wide (atomic); no
  * reference counterpart. */
 int glide_set_tile_height(int kernel, int ty);
 
-/* Levels with at most max_cells cells run their Vanka sweeps row-split (four
- * waves per 64 cells, each assembling part of the local block; shorter
- * per-wave instruction stream for latency-bound small levels); 0 disables
- * it.  A launch shape only: results are bitwise the same.  Process-wide
- * (atomic); no reference counterpart. */
-int glide_set_row_split(long max_cells);
+/* Vanka kernel of the small levels, whose sweeps are bound by the latency
+ * of one wave's Newton chain: levels with at most max_cells cells (default
+ * 128 x 128) sweep with `variant` 0 (default) = one lane per cell with the
+ * whole register file, or 1 = row split (five waves per 64 cells, each assembling
+ * one row of the local block); larger levels sweep one lane per cell at two
+ * waves per SIMD.  A launch shape only: results are bitwise the same.
+ * Process-wide (atomic); no reference counterpart. */
+int glide_set_small_levels(long max_cells, int variant);
 
 /* Copies the message of the last failure on this thread ("" if none) into
  * buf (at most cap - 1 characters and a terminating NUL; buf may be NULL

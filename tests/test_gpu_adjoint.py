@@ -102,9 +102,9 @@ def test_vjp_matches_oracle_and_dot_product(gl, shape):
 
 @pytest.fixture
 def lin_tiles():
-    """Set the JVP / VJP tile height for one test; restores the default (8)."""
+    """Set the JVP / VJP tile height for one test; restores the default (4)."""
     yield lambda ty: _lib.call("glide_set_tile_height", 1, ty)
-    _lib.call("glide_set_tile_height", 1, 8)
+    _lib.call("glide_set_tile_height", 1, 4)
 
 
 @pytest.mark.parametrize("gl", [False, True])

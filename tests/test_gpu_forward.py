@@ -114,21 +114,22 @@ def test_residual_tile_heights_bitwise(ty, shape):
 
 
 @pytest.fixture
-def row_split():
-    """Set the row-split cell limit for one test (0: every level sweeps with
-    k_vanka's tiles); restores the library default afterwards."""
-    yield lambda cells: _lib.call("glide_set_row_split", cells)
-    _lib.call("glide_set_row_split", 128 * 128)
+def small_levels():
+    """Set the small-level Vanka variant for one test; restores the library
+    default (levels up to 128 x 128 cells, one lane per cell) afterwards."""
+    yield lambda cells, variant: _lib.call("glide_set_small_levels", cells, variant)
+    _lib.call("glide_set_small_levels", 128 * 128, 0)
 
 
-# (9, 12) and (40, 70) sweep row-split by default and with k_vanka's
-# boundary and interior tiles when the split is off; 131 x 150 is above the
-# split limit, so its interior tiles take the specialised assembly.
-@pytest.mark.parametrize("split", [True, False])
+# Every shape sweeps with each kernel: the row split and the full-register
+# per-cell tile (both below the small-level limit), and the two-waves-per-SIMD
+# tile (limit 0).  131 x 150 has interior tiles, which take the specialised
+# assembly of the per-cell tiles.
+@pytest.mark.parametrize("variant", ["rows", "small", "large"])
 @pytest.mark.parametrize("shape", [(9, 12), (40, 70), (131, 150)])
-def test_vanka_sweep_bitwise(shape, split, row_split):
+def test_vanka_sweep_bitwise(shape, variant, small_levels):
     ny, nx = shape
-    row_split(128 * 128 if split else 0)
+    small_levels(*{"rows": (1 << 20, 1), "small": (1 << 20, 0), "large": (0, 0)}[variant])
     s = random_state(ny=ny, nx=nx, seed=3 * ny + nx, vel=100.0)
     prm = O.params()
     ref = O.vanka_sweep(ny, nx, s["dx"], s["dt"], prm, s["U"], s["f"], s["bed"], s["B"], s["beta"], s["gamma"])
@@ -141,9 +142,9 @@ def test_vanka_sweep_bitwise(shape, split, row_split):
     assert np.array_equal(out, ref), np.abs(out - ref).max()
 
 
-@pytest.mark.parametrize("split", [True, False])
-def test_local_activation_and_local_sweep_bitwise(split, row_split):
-    row_split(128 * 128 if split else 0)
+@pytest.mark.parametrize("variant", ["rows", "small", "large"])
+def test_local_activation_and_local_sweep_bitwise(variant, small_levels):
+    small_levels(*{"rows": (1 << 20, 1), "small": (1 << 20, 0), "large": (0, 0)}[variant])
     ny, nx = 48, 60
     s = random_state(ny=ny, nx=nx, seed=77, vel=50.0)
     prm = O.params()
