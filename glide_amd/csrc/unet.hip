@@ -105,33 +105,43 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16_t *__restrict__ xa, 
 #pragma unroll
         for (int j = 0; j < NJ; j++) acc[a][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int t = 0; t < 9; t++) {
+    // K steps s = (tap t, 32-channel chunk c): fragments of step s + 1 are
+    // loaded while the MFMAs of step s run (the accumulation order over s is
+    // the plain tap-major, channel-minor order)
+    const int nch = Cin / 32, nsteps = 9 * nch;
+    auto load = [&](int s, Frag(&A)[4], Frag(&B)[NJ]) {
+        const int t = s / nch, c0 = (s - t * nch) * 32;
         const int dy = t / 3 - 1, dx = t % 3 - 1;
-        long src[4];
-        bool ok[4];
+        const bf16_t *xs = c0 < CA ? xa : xb;
+        const int cs = c0 < CA ? CA : CB, cc = (c0 < CA ? c0 : c0 - CA) + 8 * kq;
 #pragma unroll
         for (int a = 0; a < 4; a++) {
-            int y = py[a] + dy, x = px[a] + dx;
-            ok[a] = pv[a] && y >= 0 && y < H && x >= 0 && x < W;
-            src[a] = ok[a] ? (((long)pn[a] * H + y) * W + x) : 0;
+            const int y = py[a] + dy, x = px[a] + dx;
+            const bool ok = pv[a] && y >= 0 && y < H && x >= 0 && x < W;
+            const long src = ok ? (((long)pn[a] * H + y) * W + x) : 0;
+            A[a].raw = ok ? *(const uint4 *)(xs + src * cs + cc) : uint4{0u, 0u, 0u, 0u};
         }
-        for (int c0 = 0; c0 < Cin; c0 += 32) {
-            const bf16_t *xs = c0 < CA ? xa : xb;
-            const int cs = c0 < CA ? CA : CB, cc = (c0 < CA ? c0 : c0 - CA) + 8 * kq;
-            Frag A[4];
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                A[a].raw = ok[a] ? *(const uint4 *)(xs + src[a] * cs + cc) : uint4{0u, 0u, 0u, 0u};
-            }
+        for (int j = 0; j < NJ; j++)
+            B[j].raw = *(const uint4 *)(Wt + ((long)t * Cout + co0 + 16 * j + r16) * Cin + c0 + 8 * kq);
+    };
+    auto mma = [&](const Frag(&A)[4], const Frag(&B)[NJ]) {
 #pragma unroll
-            for (int j = 0; j < NJ; j++) {
-                Frag B;
-                B.raw = *(const uint4 *)(Wt + ((long)t * Cout + co0 + 16 * j + r16) * Cin + c0 + 8 * kq);
+        for (int j = 0; j < NJ; j++)
 #pragma unroll
-                for (int a = 0; a < 4; a++)
-                    acc[a][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a].v, B.v, acc[a][j], 0, 0, 0);
-            }
-        }
+            for (int a = 0; a < 4; a++)
+                acc[a][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[a].v, B[j].v, acc[a][j], 0, 0, 0);
+    };
+    Frag Ac[4], Bc[NJ], An[4], Bn[NJ];
+    load(0, Ac, Bc);
+#pragma unroll 1
+    for (int s = 0; s < nsteps; s++) {
+        if (s + 1 < nsteps) load(s + 1, An, Bn);
+        mma(Ac, Bc);
+#pragma unroll
+        for (int a = 0; a < 4; a++) Ac[a] = An[a];
+#pragma unroll
+        for (int j = 0; j < NJ; j++) Bc[j] = Bn[j];
     }
     // D fragment: row (pixel) = 4*kq + i, column (channel) = r16
 #pragma unroll
@@ -517,6 +527,18 @@ namespace {
 inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 inline hipStream_t SS(void *s) { return (hipStream_t)s; }
 bool bad_ch(int c) { return c < 0 || c % 32 != 0 || c > 4096; }
+// Output channels per wave (16 * NJ) of a 3x3 convolution over P pixels: the
+// widest of 64, 32 and 16 that still gives at least 2048 waves (two per
+// SIMD), else 16.  Small levels are bound by each wave's serial K loop, so
+// they take more, narrower waves; large ones keep the input fragments'
+// reuse across output channels.  The sums are the same for every width.
+int conv_nj(long P, int C)
+{
+    const long pw = (long)nblk(P) * 4;   // 64-pixel waves
+    for (int nj = 4; nj > 1; nj /= 2)
+        if (C % (16 * nj) == 0 && pw * (C / (16 * nj)) >= 2048) return nj;
+    return 1;
+}
 }  // namespace
 
 extern "C" {
@@ -528,11 +550,15 @@ int glide_unet_conv3x3(const uint16_t *xa, int CA, const uint16_t *xb, int CB, i
     if (bad_ch(CA) || CA == 0 || bad_ch(CB) || bad_ch(Cout) || Cout == 0 || N < 1 || H < 1 || W < 1)
         return ufail(GLIDE_E_SIZE, "conv3x3: channels must be positive multiples of 32, sizes positive");
     const long P = (long)N * H * W;
-    if (Cout % 64 == 0)
+    const int nj = conv_nj(P, Cout);
+    if (nj == 4)
         hipLaunchKernelGGL((k_conv3x3<4, 0>), dim3(nblk(P), Cout / 64), dim3(256), 0, SS(stream), xa, CA, xb, CB, N, H,
                            W, weights, Cout, bias, relu, out, nullptr, 0, 0, nullptr, 0);
-    else
+    else if (nj == 2)
         hipLaunchKernelGGL((k_conv3x3<2, 0>), dim3(nblk(P), Cout / 32), dim3(256), 0, SS(stream), xa, CA, xb, CB, N, H,
+                           W, weights, Cout, bias, relu, out, nullptr, 0, 0, nullptr, 0);
+    else
+        hipLaunchKernelGGL((k_conv3x3<1, 0>), dim3(nblk(P), Cout / 16), dim3(256), 0, SS(stream), xa, CA, xb, CB, N, H,
                            W, weights, Cout, bias, relu, out, nullptr, 0, 0, nullptr, 0);
     return ucheck("conv3x3");
 }
@@ -545,11 +571,15 @@ int glide_unet_conv3x3_dgrad(const uint16_t *dz, int Cout, int N, int H, int W, 
         return ufail(GLIDE_E_SIZE, "conv3x3_dgrad: channels must be positive multiples of 32, sizes positive");
     const long P = (long)N * H * W;
     const int Cin = CA + CB;   // output channels of the transposed convolution
-    if (Cin % 64 == 0)
+    const int nj = conv_nj(P, Cin);
+    if (nj == 4)
         hipLaunchKernelGGL((k_conv3x3<4, 1>), dim3(nblk(P), Cin / 64), dim3(256), 0, SS(stream), dz, Cout, nullptr, 0,
                            N, H, W, wflip, Cin, nullptr, 0, nullptr, ga, CA, acc_a, gb, acc_b);
-    else
+    else if (nj == 2)
         hipLaunchKernelGGL((k_conv3x3<2, 1>), dim3(nblk(P), Cin / 32), dim3(256), 0, SS(stream), dz, Cout, nullptr, 0,
+                           N, H, W, wflip, Cin, nullptr, 0, nullptr, ga, CA, acc_a, gb, acc_b);
+    else
+        hipLaunchKernelGGL((k_conv3x3<1, 1>), dim3(nblk(P), Cin / 16), dim3(256), 0, SS(stream), dz, Cout, nullptr, 0,
                            N, H, W, wflip, Cin, nullptr, 0, nullptr, ga, CA, acc_a, gb, acc_b);
     return ucheck("conv3x3_dgrad");
 }
