@@ -59,18 +59,23 @@ inline hipStream_t S(void *s) { return (hipStream_t)s; }
 // Workgroups per CU of the persistent single-level launches (Vanka sweep,
 // residual); set by glide_set_persistence.  The Vanka sweep's 64 gives one
 // tile per workgroup at every level size used here (measured fastest: the
-// hardware then balances clustered local-smoother work); the residual's 4
-// loops each workgroup over several tiles.  Launch shapes only: results do
-// not depend on them.
+// hardware then balances clustered local-smoother work); the residual's 8
+// loops each workgroup over several tiles where a level has more tiles than
+// that.  Launch shapes only: results do not depend on them.
 std::atomic<int> g_wg_per_cu[2] = {{64}, {8}};
-// Levels with at most this many cells use the row-split Vanka sweep.
+// Levels with at most g_small_max_cells cells sweep with the small-level
+// kernel: variant 0 = k_vanka compiled for one wave per SIMD (the whole
+// register file), 1 = the row split (k_vanka_rows); set by
+// glide_set_small_levels.  Measured (sweep + apply, 10 Newton iterations):
+// 32^2 22.5 / 23.2 us, 64^2 23.6 / 24.0, 128^2 23.4 / 24.1 (variant 0 / 1).
 std::atomic<long> g_small_max_cells{128 * 128};
 std::atomic<int> g_small_variant{0};
 // Tile heights (64 x TY points) of the single-level residual (4, 8, 16; 0 =
-// by level size) and of the JVP / VJP (0 = the per-point form, 4, 8); set by
-// glide_set_tile_height.  Launch shapes only: results are the same bits.
+// by level size) and of the JVP / VJP (0 = the per-point form, 4, 6, 8); set
+// by glide_set_tile_height.  Launch shapes only: results are the same bits.
 // Measured at 512^2: residual 64 x 16 tiles 11.3 us, 64 x 8 12.0, 64 x 4 13.4;
 // at 1024^2: 64 x 8 39.7 us, 64 x 16 42.0, 64 x 4 46.4 (8 workgroups per CU).
+// VJP at 512^2: 64 x 4 22.5 us, 64 x 8 26.1, 64 x 6 31.8, per point 39.6.
 std::atomic<int> g_res_ty{0};
 std::atomic<int> g_lin_ty{4};
 int residual_ty(int ny, int nx)
