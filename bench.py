@@ -10,7 +10,8 @@ of glide_amd/data/warmstart_unet.npz; inference inside the timed region)
 are timed on the same seeds; value = the learned time (plain if no weights).
 (dt = 1 a: with dt = 10 a plain FASCD stalls above 1e-8 on most 512^2 sheets
 of this family, so a time to 1e-8 does not exist there.)
-One "step" = one seed's forward step; inputs are resident in HBM.
+One "step" = one seed's forward step (20 seeds by default, 20001-20020, after
+the warm-up seed 20000); inputs are resident in HBM.
 
 Also reported on the same JSON line:
   - learned_batched: 16 such sheets as one ensemble, plain vs learned;
@@ -45,7 +46,7 @@ from glide_amd.roofline import (FLOPS_NEWTON_ITER, PEAK_BF16_TFLOPS, PEAK_FP64_T
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4, help="timed forward steps (one test seed each)")
+    ap.add_argument("--steps", type=int, default=20, help="timed forward steps (one test seed each)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cap_512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
