@@ -587,8 +587,9 @@ def inversion_bench(n=512, lbfgs_iters=8):
     (synthetic.ice_slab, seed 2) observed through a seeded truth beta with 2%
     noise, L2 misfit plus 1e-3 Tikhonov on log beta.  Reports one adjoint
     V-cycle, one objective + gradient evaluation when converged (forward and
-    adjoint to 1e-10) and on the reference's fixed schedule (5 forward
-    V-cycles, 1 adjoint V-cycle), and a short L-BFGS run."""
+    adjoint to 1e-10; on the tolerance schedule and with the reference's
+    sweep counts) and on the reference's fixed schedule (5 forward V-cycles,
+    1 adjoint V-cycle), and a short L-BFGS run on the tolerance schedule."""
     import numpy as np
     import torch
 
@@ -608,8 +609,11 @@ def inversion_bench(n=512, lbfgs_iters=8):
     x0 = np.full(n * n, np.log(float(np.mean(truth))))
     out = {"workload": f"beta inversion: {n}x{n} grounded slab, dx=1500 m, 5 levels, dt=10 a, L2 misfit + "
                        "1e-3 Tikhonov on log beta, observations = truth solve + 2% noise"}
-    modes = (("converged", dict(mode="l2", fwd_tol=1e-10, adj_tol=1e-10)), ("reference_schedule", dict(mode="reference")))
-    for name, kw in modes:
+    modes = (("converged", dict(mode="l2", fwd_tol=1e-10, adj_tol=1e-10), TS),
+             ("converged_reference_sweeps", dict(mode="l2", fwd_tol=1e-10, adj_tol=1e-10), {}),
+             ("reference_schedule", dict(mode="reference"), {}))
+    for name, kw, sched in modes:
+        ph.set_schedule(**sched)
         inv = BetaInversion(ph, u_obs, v_obs, w["H"], dt=10.0, alpha=1e-3, **kw)
         inv.objective(x0)
         reps = 3
@@ -619,15 +623,18 @@ def inversion_bench(n=512, lbfgs_iters=8):
             inv.objective(x0)
         torch.cuda.synchronize()
         out[f"{name}_ms_per_evaluation"] = round((time.perf_counter() - t0) / reps * 1e3, 2)
-        if name == "converged":
-            out["converged_vcycles"] = {"forward": len(ph.last_history), "adjoint": len(ph.last_adj_history)}
-    reps = 20
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        ph.adjoint_vcycle()
-    torch.cuda.synchronize()
-    out["ms_per_adjoint_vcycle"] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+        if name.startswith("converged"):
+            out[f"{name}_vcycles"] = {"forward": len(ph.last_history), "adjoint": len(ph.last_adj_history)}
+    for name, sched in (("ms_per_adjoint_vcycle", TS), ("ms_per_adjoint_vcycle_reference_sweeps", {})):
+        ph.set_schedule(**sched)
+        reps = 20
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ph.adjoint_vcycle()
+        torch.cuda.synchronize()
+        out[name] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+    ph.set_schedule(**TS)
     inv = BetaInversion(ph, u_obs, v_obs, w["H"], dt=10.0, alpha=1e-3, mode="l2", fwd_tol=1e-10, adj_tol=1e-10)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -134,3 +134,36 @@ def test_inversion_reduces_misfit():
     m0, m1 = inv.history[0]["misfit"], inv.history[-1]["misfit"]
     print("misfit", m0, "->", m1, "iterations", res.nit, flush=True)
     assert m1 < 0.05 * m0
+
+
+def test_tolerance_schedule_gives_the_reference_sweeps_gradient():
+    """The inversion's converged gradient does not depend on the coarsest
+    levels' sweep counts: forward to 1e-12 and adjoint to 1e-12 on a 128^2
+    slab with the tolerance schedule (5 forward, 10 adjoint coarsest sweeps,
+    boundary weight 0.5) and with the reference's counts (200, 100, weight 1)
+    give the same misfit gradient to 1e-9, with the adjoint taking the same
+    number of V-cycles."""
+    from glide_amd.physics import TOLERANCE_SCHEDULE
+
+    n = 128
+    w = S.ice_slab(n, n, 1500.0, seed=2)
+    truth = S.ice_slab(n, n, 1500.0, seed=3)["beta"]
+    ph = IcePhysics(n, n, w["dx"], n_levels=4, gl_derivatives=False)
+    ph.set_geometry(w["bed"], w["H"])
+    ph.set_parameters(B=w["B"], beta=truth, smb=w["smb"])
+    ph.forward(10.0, tol=1e-12, max_vcycles=80, update_geometry=False)
+    rng = np.random.default_rng(3)
+    u_obs = ph.grid.u.cpu().numpy() * (1 + 0.02 * rng.standard_normal((n, n + 1)))
+    v_obs = ph.grid.v.cpu().numpy() * (1 + 0.02 * rng.standard_normal((n + 1, n)))
+    x0 = np.full(n * n, np.log(float(np.mean(truth))))
+    out = {}
+    for name, sched in (("reference", {}), ("tolerance", TOLERANCE_SCHEDULE)):
+        ph.set_schedule(**sched)
+        inv = BetaInversion(ph, u_obs, v_obs, w["H"], dt=10.0, alpha=1e-3, mode="l2", fwd_tol=1e-12, adj_tol=1e-12)
+        J, g = inv.objective(x0)
+        out[name] = (J, g, len(ph.last_history), len(ph.last_adj_history))
+        print(name, "forward", out[name][2], "adjoint", out[name][3], "V-cycles", flush=True)
+    (Jr, gr, _, ar), (Jt, gt, _, at) = out["reference"], out["tolerance"]
+    assert abs(Jt - Jr) <= 1e-10 * abs(Jr)
+    assert np.linalg.norm(gt - gr) <= 1e-9 * np.linalg.norm(gr)
+    assert at == ar
