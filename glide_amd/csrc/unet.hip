@@ -332,6 +332,50 @@ __global__ void k_pool2_bwd(const float *__restrict__ dy, int N, int H, int W, i
     const float g = 0.25f * dy[(((long)n * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
     dx[k] = acc ? dx[k] + g : g;
 }
+// 8 channels in one 16-byte access
+union Bf8 {
+    uint4 raw;
+    bf16_t h[8];
+};
+
+// 2x2 average pooling over 8-channel groups (C % 8 == 0, 16-byte aligned
+// tensors): per channel the same sums as k_pool2
+__global__ void k_pool2_v8(const bf16_t *__restrict__ x, int N, int H, int W, int C, bf16_t *__restrict__ y)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    if (k >= (long)N * Ho * Wo * C8) return;
+    const int c8 = (int)(k % C8);
+    long r = k / C8;
+    const int xo = (int)(r % Wo);
+    r /= Wo;
+    const int yo = (int)(r % Ho), n = (int)(r / Ho);
+    const long b = (((long)n * H + 2 * yo) * W + 2 * xo) * C + 8 * c8;
+    Bf8 q0, q1, q2, q3, o;
+    q0.raw = *(const uint4 *)(x + b);
+    q1.raw = *(const uint4 *)(x + b + C);
+    q2.raw = *(const uint4 *)(x + b + (long)W * C);
+    q3.raw = *(const uint4 *)(x + b + (long)W * C + C);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const float s = ((bf2f(q0.h[i]) + bf2f(q1.h[i])) + bf2f(q2.h[i])) + bf2f(q3.h[i]);
+        o.h[i] = f2bf(0.25f * s);
+    }
+    *(uint4 *)(y + 8 * k) = o.raw;
+}
+// nearest x2 upsampling over 8-channel groups (C % 8 == 0, aligned)
+__global__ void k_up2_v8(const bf16_t *__restrict__ x, int N, int H, int W, int C, bf16_t *__restrict__ y)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int C8 = C / 8;
+    if (k >= (long)N * 4 * H * W * C8) return;
+    const int c8 = (int)(k % C8);
+    long r = k / C8;
+    const int xo = (int)(r % (2 * W));
+    r /= 2 * W;
+    const int yo = (int)(r % (2 * H)), n = (int)(r / (2 * H));
+    *(uint4 *)(y + 8 * k) = *(const uint4 *)(x + (((long)n * H + yo / 2) * W + xo / 2) * C + 8 * c8);
+}
 // nearest x2 upsampling, NHWC (input N x H x W)
 __global__ void k_up2(const bf16_t *__restrict__ x, int N, int H, int W, int C, bf16_t *__restrict__ y)
 {
@@ -381,6 +425,26 @@ __global__ void k_head(const bf16_t *__restrict__ x, long P, int C, int NO, cons
     for (int c = 0; c < C; c++) {
         float v = bf2f(x[p * C + c]);
         for (int o = 0; o < NO; o++) s[o] += Wh[o * C + c] * v;
+    }
+    for (int o = 0; o < NO; o++) out[NO * p + o] = s[o];
+}
+// k_head reading 8 channels per 16-byte access (C % 8 == 0, aligned): the
+// same sums in the same order
+__global__ void k_head_v8(const bf16_t *__restrict__ x, long P, int C, int NO, const float *__restrict__ Wh,
+                          const float *__restrict__ bh, float *__restrict__ out)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int o = 0; o < NO; o++) s[o] = bh[o];
+    for (int c = 0; c < C; c += 8) {
+        Bf8 q;
+        q.raw = *(const uint4 *)(x + p * C + c);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float v = bf2f(q.h[i]);
+            for (int o = 0; o < NO; o++) s[o] += Wh[o * C + c + i] * v;
+        }
     }
     for (int o = 0; o < NO; o++) out[NO * p + o] = s[o];
 }
@@ -527,6 +591,8 @@ namespace {
 inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 inline hipStream_t SS(void *s) { return (hipStream_t)s; }
 bool bad_ch(int c) { return c < 0 || c % 32 != 0 || c > 4096; }
+// 8-channel (16-byte) accesses apply: C % 8 == 0 and both tensors 16-byte aligned
+bool vec8(int C, const void *a, const void *b) { return C % 8 == 0 && (((uintptr_t)a | (uintptr_t)b) & 15) == 0; }
 // Output channels per wave (16 * NJ) of a 3x3 convolution over P pixels: the
 // widest of 64, 32 and 16 that still gives at least 2048 waves (two per
 // SIMD), else 16.  Small levels are bound by each wave's serial K loop, so
@@ -635,7 +701,10 @@ int glide_unet_pool2(const uint16_t *x, int N, int H, int W, int C, uint16_t *y,
     if (!x || !y) return ufail(GLIDE_E_NULL, "pool2: NULL argument");
     if (N < 1 || H < 2 || W < 2 || H % 2 || W % 2 || C < 1) return ufail(GLIDE_E_SIZE, "pool2: even sizes required");
     const long n = (long)N * (H / 2) * (W / 2) * C;
-    hipLaunchKernelGGL(k_pool2, dim3(nblk(n)), dim3(256), 0, SS(stream), x, N, H, W, C, y);
+    if (vec8(C, x, y))
+        hipLaunchKernelGGL(k_pool2_v8, dim3(nblk(n / 8)), dim3(256), 0, SS(stream), x, N, H, W, C, y);
+    else
+        hipLaunchKernelGGL(k_pool2, dim3(nblk(n)), dim3(256), 0, SS(stream), x, N, H, W, C, y);
     return ucheck("pool2");
 }
 
@@ -653,7 +722,10 @@ int glide_unet_up2(const uint16_t *x, int N, int H, int W, int C, uint16_t *y, v
     if (!x || !y) return ufail(GLIDE_E_NULL, "up2: NULL argument");
     if (N < 1 || H < 1 || W < 1 || C < 1) return ufail(GLIDE_E_SIZE, "up2: bad size");
     const long n = (long)N * 4 * H * W * C;
-    hipLaunchKernelGGL(k_up2, dim3(nblk(n)), dim3(256), 0, SS(stream), x, N, H, W, C, y);
+    if (vec8(C, x, y))
+        hipLaunchKernelGGL(k_up2_v8, dim3(nblk(n / 8)), dim3(256), 0, SS(stream), x, N, H, W, C, y);
+    else
+        hipLaunchKernelGGL(k_up2, dim3(nblk(n)), dim3(256), 0, SS(stream), x, N, H, W, C, y);
     return ucheck("up2");
 }
 
@@ -684,7 +756,10 @@ int glide_unet_head(const uint16_t *x, long P, int C, int NO, const float *wh, c
 {
     if (!x || !wh || !bh || !out) return ufail(GLIDE_E_NULL, "head: NULL argument");
     if (P < 1 || C < 1 || NO < 1 || NO > 4) return ufail(GLIDE_E_SIZE, "head: bad size (1 <= outputs <= 4)");
-    hipLaunchKernelGGL(k_head, dim3(nblk(P)), dim3(256), 0, SS(stream), x, P, C, NO, wh, bh, out);
+    if (vec8(C, x, x))
+        hipLaunchKernelGGL(k_head_v8, dim3(nblk(P)), dim3(256), 0, SS(stream), x, P, C, NO, wh, bh, out);
+    else
+        hipLaunchKernelGGL(k_head, dim3(nblk(P)), dim3(256), 0, SS(stream), x, P, C, NO, wh, bh, out);
     return ucheck("head");
 }
 

@@ -118,9 +118,11 @@ def test_conv_gradients_match_oracle():
     assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
 
 
-def test_elementwise_layers():
-    rng = np.random.default_rng(6)
-    N, H, W, C = 2, 8, 6, 32
+@pytest.mark.parametrize("C", [32, 4])
+def test_pool_and_upsample_bitwise(C):
+    """C = 32 takes the 8-channel vector kernels, C = 4 the per-element ones."""
+    rng = np.random.default_rng(5 + C)
+    N, H, W = 2, 8, 6
     x = bf16_rand(rng, (N, H, W, C))
     y = torch.empty(N * H * W * C // 4, dtype=torch.int16, device=DEV)
     _lib.call("glide_unet_pool2", P(bits_t(x)), N, H, W, C, P(y), S())
@@ -128,6 +130,12 @@ def test_elementwise_layers():
     up = torch.empty(N * H * W * C * 4, dtype=torch.int16, device=DEV)
     _lib.call("glide_unet_up2", P(bits_t(x)), N, H, W, C, P(up), S())
     assert np.array_equal(from_bits_t(up, (N, 2 * H, 2 * W, C)), U.up2(x))
+
+
+def test_elementwise_layers():
+    rng = np.random.default_rng(6)
+    N, H, W, C = 2, 8, 6, 32
+    x = bf16_rand(rng, (N, H, W, C))
     dy = rng.standard_normal((N, H // 2, W // 2, C)).astype(np.float32)
     dx = torch.full((N * H * W * C,), 2.0, dtype=torch.float32, device=DEV)
     _lib.call("glide_unet_pool2_bwd", P(f32_t(dy)), N, H, W, C, P(dx), 1, S())
@@ -195,3 +203,20 @@ def test_training_reduces_loss():
         net.adam_step(2e-3)
     print("loss", losses[0], "->", losses[-1], flush=True)
     assert losses[-1] < losses[0] / 5
+
+
+@pytest.mark.parametrize("C", [32, 12])
+def test_head_matches_oracle(C):
+    """The 1x1 head (FP32 sums over BF16 inputs) against the oracle's float64
+    sums; C = 32 takes the 8-channel vector kernel, C = 12 the per-element
+    one (both sum the channels in ascending order)."""
+    rng = np.random.default_rng(40 + C)
+    Pn, NO = 1000, 3
+    x = bf16_rand(rng, (Pn, C))
+    Wh = rng.standard_normal((NO, C)).astype(np.float32) * 0.2
+    bh = rng.standard_normal(NO).astype(np.float32)
+    out = torch.empty(Pn * NO, dtype=torch.float32, device=DEV)
+    _lib.call("glide_unet_head", P(bits_t(x)), Pn, C, NO, P(f32_t(Wh)), P(f32_t(bh)), P(out), S())
+    ref = U.head(x, Wh, bh)
+    got = out.cpu().numpy().reshape(Pn, NO)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
