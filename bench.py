@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--no-1024", action="store_true", help="skip the 1024^2 warm-start solves")
     ap.add_argument("--ensemble-only", action="store_true", help="only the 64-member batched ensemble")
     ap.add_argument("--only-1024", action="store_true", help="only the 1024^2 warm-start solves")
+    ap.add_argument("--unet-only", action="store_true", help="only the warm-start network's inference rooflines")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline solves and the kernel rooflines (for profiling them)")
     ap.add_argument("--cpu-report", action="store_true",
@@ -688,6 +689,11 @@ def main():
         return
     if args.ensemble_only:
         print(json.dumps(ensemble_bench()))
+        return
+    if args.unet_only:
+        from glide_amd.warmstart import UNet
+        net = UNet.load(WEIGHTS)
+        print(json.dumps({f"unet_roofline_{n}": unet_roofline(net, n=n, reps=20) for n in (512, 1024)}))
         return
     if args.only_1024:
         from glide_amd.warmstart import UNet
