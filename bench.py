@@ -77,6 +77,12 @@ def parse():
 # WRITE 10,240 + 6,248; residual (cell tiles, 64 x 16): FETCH 14,137.4, WRITE
 # 6,248; JVP / VJP (64 x 4 tiles, XCD-chunked order): FETCH 11,717.3 / 11,872.0,
 # WRITE 6,179.7 / 6,182.1.
+# HBM bytes per launch of the batched residual restriction over phase 0 of the
+# 64-member plan (16 x 768^2 + 16 x 512^2 finest levels -> their children),
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on bench.py --ensemble-only with the
+# calibration above: FETCH 135,703.5, WRITE 81,546.5 KiB per dispatch (the
+# compulsory 409 MB less the part of r the 256 MB last-level cache serves).
+TRAFFIC_ENSEMBLE_RESTRICT_PHASE0 = (2 * 135703.5 + 81546.5) * 1024
 TRAFFIC_512 = {"vanka": (2 * (13209.6 + 8710.5) + 10240.0 + 6248.0) * 1024,
                "residual": (2 * 14137.4 + 6248.0) * 1024,
                "jvp": (2 * 11717.3 + 6179.7) * 1024,
@@ -312,6 +318,10 @@ def ensemble_bench(dt=10.0, n_vcycles=3):
     res_ms = ens.time_kernel(BK_RESIDUAL, fin, ty, 0, reps=20)
     res_bytes = sum(residual_bytes(d["ny"], d["nx"]) for d in members)
     res_gbs = res_bytes / (res_ms * 1e-3) / 1e9
+    # the batched residual restriction (pure memory traffic) of the phase that
+    # moves the most bytes
+    tr = max((ens.time_restriction(p, reps=20) + (p,) for p in range(ens.n_phases)), key=lambda x: x[1])
+    tr_gbs = tr[1] / (tr[0] * 1e-3) / 1e9
     cells = sum(d["ny"] * d["nx"] for d in members)
     solo = np.array(solo_ms)
     by_size = {}
@@ -340,6 +350,12 @@ def ensemble_bench(dt=10.0, n_vcycles=3):
                              "achieved": round(res_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": round(res_gbs / PEAK_HBM_GBS, 4), "ms_per_launch": round(res_ms, 4),
                              "work": f"{res_bytes} algorithmic bytes", "tile_height": ty},
+        "transfer_roofline": {"kernel": f"kb_point<OP_RESTRICT_R> (residual restriction, phase {tr[2]}: every "
+                                        "member with a child level, one launch)", "bound": "hbm",
+                              "achieved": round(tr_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                              "frac": round(tr_gbs / PEAK_HBM_GBS, 4), "ms_per_launch": round(tr[0], 4),
+                              "traffic": TRAFFIC_ENSEMBLE_RESTRICT_PHASE0 if tr[2] == 0 else None,
+                              "work": f"{tr[1]:.0f} compulsory bytes (8 x (N_total + N_total,child) per member)"},
     }
     ens.close()
     del phs

@@ -106,6 +106,7 @@ SIGNATURES = {
     "glide_batch_set_enabled": (c_int, [c_void_p, POINTER(ctypes.c_ubyte), c_void_p]),
     "glide_batch_set_tiles": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "glide_batch_time_kernel": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, POINTER(c_double), c_void_p]),
+    "glide_batch_time_restriction": (c_int, [c_void_p, c_int, c_int, POINTER(c_double), POINTER(c_double), c_void_p]),
     "glide_unet_conv3x3": (c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p,
                                    c_int, c_void_p, c_void_p]),
     "glide_unet_conv3x3_dgrad": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int,

@@ -835,4 +835,42 @@ int glide_batch_time_kernel(GlideBatch *b, int kernel, int phase, int ty, int wg
     return rc;
 }
 
+int glide_batch_time_restriction(GlideBatch *b, int phase, int reps, double *ms, double *bytes, void *stream)
+{
+    if (!b || !ms || !bytes) return bfail(GLIDE_E_NULL, "batch, ms and bytes are required");
+    if (reps < 1 || reps > 1000) return bfail(GLIDE_E_ARG, "reps out of range");
+    if (phase < 0 || phase >= b->n_phases) return bfail(GLIDE_E_ARG, "phase out of range");
+    const BPhase &P = b->ph[phase];
+    double nb = 0.0;
+    for (int e = P.off; e < P.off + P.n; e++) {
+        const BEntry &E = b->h_entries[e];
+        if (!E.has_child) continue;
+        const size_t nf = n_u(E.L.g.ny, E.L.g.nx) + n_v(E.L.g.ny, E.L.g.nx) + n_c(E.L.g.ny, E.L.g.nx);
+        const size_t nc = n_u(E.C.g.ny, E.C.g.nx) + n_v(E.C.g.ny, E.C.g.nx) + n_c(E.C.g.ny, E.C.g.nx);
+        nb += 8.0 * (double)(nf + nc);
+    }
+    *bytes = nb;
+    *ms = 0.0;
+    if (nb == 0.0) return GLIDE_OK;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    hipError_t he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he != hipSuccess) return bfail((int)he, "event creation failed");
+    int rc = b_point<OP_RESTRICT_R>(b, phase, 1.0, st);   // warm-up
+    if (rc == GLIDE_OK) (void)hipEventRecord(e0, st);
+    for (int k = 0; k < reps && rc == GLIDE_OK; k++) rc = b_point<OP_RESTRICT_R>(b, phase, 1.0, st);
+    if (rc == GLIDE_OK) {
+        (void)hipEventRecord(e1, st);
+        he = hipEventSynchronize(e1);
+        float t = 0.0f;
+        if (he == hipSuccess) he = hipEventElapsedTime(&t, e0, e1);
+        if (he != hipSuccess) rc = bfail((int)he, "event timing failed");
+        *ms = (double)t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 }  // extern "C"

@@ -105,6 +105,14 @@ class Ensemble:
                   ctypes.byref(ms), self._stream())
         return ms.value
 
+    def time_restriction(self, phase: int, reps: int = 20) -> tuple[float, float]:
+        """(ms per launch, compulsory bytes) of the batched residual
+        restriction of one phase (every member with a child level)."""
+        ms, nb = ctypes.c_double(), ctypes.c_double()
+        _lib.call("glide_batch_time_restriction", self._h, int(phase), int(reps), ctypes.byref(ms), ctypes.byref(nb),
+                  self._stream())
+        return ms.value, nb.value
+
     # ----------------------------------------------------------- solve
     def forward_setup(self, dt: float):
         for m in self.members:

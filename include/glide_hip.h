@@ -328,6 +328,17 @@ int glide_batch_set_tiles(GlideBatch *batch, int kernel, int phase, int ty, int 
 int glide_batch_time_kernel(GlideBatch *batch, int kernel, int phase, int ty, int wg_per_cu,
                             int reps, double *ms, void *stream);
 
+/* Times the batched restriction of the residual (r_c = R r, every member of
+ * `phase` that has a child level, one launch) over reps launches on the
+ * stream: *ms per launch and *bytes = 8 (N_total + N_total,child) summed
+ * over those members, the compulsory traffic.  Writes the child levels' r
+ * only (a V-cycle recomputes it before use).  The restriction is the
+ * reference's residual transfer (glide/solver.py:119-121 with
+ * glide/cuda/utility.cu:1-19,125-143), here as a benchmark hook: no
+ * reference counterpart. */
+int glide_batch_time_restriction(GlideBatch *batch, int phase, int reps, double *ms, double *bytes,
+                                 void *stream);
+
 /* Workgroups per CU of the persistent single-level launches: kernel 0 =
  * the Vanka sweep (default 64, i.e. one tile per workgroup), 1 = the
  * residual (default 8).  Each launch has

@@ -132,6 +132,37 @@ def test_disabled_members_are_untouched(members):
     ens.close()
 
 
+def test_restriction_timing_hook(members):
+    """glide_batch_time_restriction counts 8 (N_total + N_total,child) per
+    member with a child in the phase, returns a positive time, and leaves
+    every member's state untouched (it writes the children's r only)."""
+    batch = [make(d) for d in members]
+    ens = Ensemble(batch)
+    ens.set_schedule(**SCHED)
+    ens.forward(10.0, n_vcycles=1)
+    torch.cuda.synchronize()
+    before = [state(ph) for ph in batch]
+
+    def ntot(ny, nx):
+        return ny * (nx + 1) + (ny + 1) * nx + ny * nx
+
+    for p in range(ens.n_phases):
+        ms, nb = ens.time_restriction(p, reps=3)
+        # member m's level k sits in phase k + (L_max - L_m); it has a child if k < L_m - 1
+        expect = 0
+        for d in members:
+            k = p - (ens.n_phases - d["n_levels"])
+            if 0 <= k < d["n_levels"] - 1:
+                ny, nx = d["ny"] >> k, d["nx"] >> k
+                expect += 8 * (ntot(ny, nx) + ntot(ny // 2, nx // 2))
+        assert nb == expect, (p, nb, expect)
+        assert (ms > 0) == (expect > 0)
+    torch.cuda.synchronize()
+    for k, ph in enumerate(batch):
+        assert np.array_equal(before[k], state(ph)), f"member {k}"
+    ens.close()
+
+
 def test_tolerance_mode_stops_converged_members():
     # ice-covered slabs converge; each member stops at its own V-cycle
     sizes = [(32, 32), (64, 64), (64, 32)]
