@@ -28,6 +28,9 @@ def _setup(n=32, levels=3, seed=2, gl=True):
     ph = IcePhysics(n, n, w["dx"], n_levels=levels, gl_derivatives=gl)
     ph.set_geometry(w["bed"], w["H"])
     ph.set_parameters(B=w["B"], beta=w["beta"], smb=w["smb"])
+    # the checks need converged solves, not the reference's sweep counts:
+    # 20 coarsest sweeps (forward and adjoint) converge as fast, for less time
+    ph.set_schedule(coarse=20, adj_coarse=20)
     return w, ph
 
 
@@ -167,3 +170,41 @@ def test_tolerance_schedule_gives_the_reference_sweeps_gradient():
     assert abs(Jt - Jr) <= 1e-10 * abs(Jr)
     assert np.linalg.norm(gt - gr) <= 1e-9 * np.linalg.norm(gr)
     assert at == ar
+
+
+def test_two_chained_steps_gradient_matches_finite_differences():
+    """Two implicit steps chained through autograd: step 2 starts from step
+    1's thickness, beta acts in both, and the loss sees only step 2's state.
+    The gradient flows back through SSAStep twice (dL/dH_prev of step 2 into
+    dL/dH of step 1) and must match 4th-order central differences of the
+    twice-solved loss in 6 random beta directions."""
+    w, ph = _setup(gl=False)
+    cfg = SolveConfig(dt=10.0, tol=1e-14, max_vcycles=120, adj_tol=1e-14, max_adj_vcycles=400)
+    rng = np.random.default_rng(11)
+    ny = nx = 32
+    u_obs = torch.as_tensor(rng.normal(0, 30.0, (ny, nx + 1)), device=DEV)
+    v_obs = torch.as_tensor(rng.normal(0, 30.0, (ny + 1, nx)), device=DEV)
+    wH = torch.as_tensor(rng.normal(0, 1.0, (ny, nx)), device=DEV)
+    H0 = torch.as_tensor(w["H"], device=DEV)
+
+    def loss_of(beta):
+        _, _, H1 = ssa_step(ph, beta, H0, cfg)
+        u2, v2, H2 = ssa_step(ph, beta, H1, cfg)
+        return 0.5 * ((u2 - u_obs) ** 2).sum() + 0.5 * ((v2 - v_obs) ** 2).sum() + (wH * H2).sum()
+
+    beta = torch.as_tensor(w["beta"], device=DEV).clone().requires_grad_(True)
+    loss_of(beta).backward()
+    gb = beta.grad.detach().clone()
+    worst = 0.0
+    with torch.no_grad():
+        for k in range(6):
+            e = torch.as_tensor(rng.normal(size=(ny, nx)), device=DEV) * beta.detach()
+            h = 1e-4
+            L1p, L1m = float(loss_of(beta.detach() + h * e)), float(loss_of(beta.detach() - h * e))
+            L2p, L2m = float(loss_of(beta.detach() + 2 * h * e)), float(loss_of(beta.detach() - 2 * h * e))
+            fd = (8.0 * (L1p - L1m) - (L2p - L2m)) / (12.0 * h)
+            ad = float((gb * e).sum())
+            rel = abs(fd - ad) / abs(fd)
+            worst = max(worst, rel)
+            print(f"direction {k}: adjoint {ad:.10e} fd {fd:.10e} rel {rel:.2e}", flush=True)
+    assert worst <= 1e-6, worst
