@@ -574,16 +574,17 @@ def learned_batched_bench(net, seeds, n_members=16, dt=DT):
     return out
 
 
-def reference_schedule_bench(dt=10.0, steps=3):
-    """The parity workload cap_512 (ice cap with a floating margin) on the
-    reference's fixed step of 3 V-cycles: ms per step and the relative
-    residual reached (plain FASCD stalls near 1e-3 on floating margins)."""
+def reference_schedule_bench(name="cap_512", steps=3):
+    """A parity workload on the reference's fixed step of 3 V-cycles: ms per
+    step and per V-cycle and the relative residual reached.  cap_512 is the
+    ice cap with a floating margin (plain FASCD stalls near 1e-3 there),
+    dome_128 the 128^2 dome."""
     import torch
 
     from glide_amd import synthetic as S
     from glide_amd.physics import IcePhysics
 
-    w = S.workload("cap_512")
+    w = S.workload(name)
     ph = IcePhysics(w["ny"], w["nx"], w["dx"], n_levels=w["n_levels"])
     ph.set_geometry(w["bed"], w["H"])
     ph.set_parameters(B=w["B"], beta=w["beta"], smb=w["smb"])
@@ -595,8 +596,10 @@ def reference_schedule_bench(dt=10.0, steps=3):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / steps * 1e3
     ph.forward(w["dt"], n_vcycles=w["n_vcycles"], tol=1e-300, max_vcycles=w["n_vcycles"])
-    return {"workload": "cap_512: 512x512, dx=1500 m, 5 levels, dt=10 a, 3 V-cycles/step, floating margin",
-            "ms_per_step": round(ms, 3), "rel_residual_after_step": [float(f"{x:.3e}") for x in ph.last_history]}
+    desc = {"cap_512": "cap_512: 512x512, dx=1500 m, 5 levels, dt=10 a, 3 V-cycles/step, floating margin",
+            "dome_128": "dome_128: 128x128, dx=2000 m, 4 levels, dt=10 a, 3 V-cycles/step"}[name]
+    return {"workload": desc, "ms_per_step": round(ms, 3), "ms_per_vcycle": round(ms / w["n_vcycles"], 3),
+            "rel_residual_after_step": [float(f"{x:.3e}") for x in ph.last_history]}
 
 
 def inversion_bench(n=512, lbfgs_iters=8):
@@ -770,6 +773,7 @@ def main():
         "all_converged": not stalled,
         "vcycles_plain": [r["vcycles_plain"] for r in rows],
         "vcycles_learned": [r.get("vcycles_learned") for r in rows],
+        "plain_ms_per_vcycle": r2(float(np.mean([r["plain_ms"] / r["vcycles_plain"] for r in both]))) if both else None,
         "fallbacks": int(sum(bool(r.get("fallback")) for r in rows)),
         "roofline": finest,
         "small_level_sweep_roofline": coarse,
@@ -784,7 +788,8 @@ def main():
         if not args.no_1024:
             line["learned_1024"] = learned_1024_bench(net)
     line["learned_batched"] = learned_batched_bench(net, [r["seed"] for r in rows])
-    line["reference_schedule"] = reference_schedule_bench()
+    line["reference_schedule"] = reference_schedule_bench("cap_512")
+    line["reference_schedule_dome_128"] = reference_schedule_bench("dome_128")
     if not args.no_inversion:
         line["inversion"] = inversion_bench()
     if not args.no_ensemble:
