@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--ensemble-only", action="store_true", help="only the 64-member batched ensemble")
     ap.add_argument("--only-1024", action="store_true", help="only the 1024^2 warm-start solves")
     ap.add_argument("--unet-only", action="store_true", help="only the warm-start network's inference rooflines")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="only the continuously batched solves (slots refilled at 1, 4, 8 and 16 free)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline solves and the kernel rooflines (for profiling them)")
     ap.add_argument("--cpu-report", action="store_true",
@@ -574,6 +576,59 @@ def learned_batched_bench(net, seeds, n_members=16, dt=DT):
     return out
 
 
+def learned_stream_bench(net, n_slots=16, n_jobs=48, dt=DT, refill_min=1):
+    """Continuous batching (glide_amd.ensemble.stream_solve): n_jobs one-step
+    jobs on the headline's 512^2 sheets (test seeds 20001...) through
+    n_slots batched slots, a slot taking the next job as soon as its job
+    reaches 1e-8; plain and with the learned warm start.  The jobs' inputs
+    are staged in HBM before the timed region; ms per member-step = wall
+    time / jobs."""
+    import numpy as np
+    import torch
+
+    from glide_amd import synthetic as S
+    from glide_amd.ensemble import Ensemble, stream_solve
+    from glide_amd.train_warmstart import TEST_SEED0, member
+
+    dev = torch.device("cuda")
+    staged = []
+    for k in range(n_jobs):
+        d = S.ice_sheet(N512, N512, DX512, TEST_SEED0 + 1 + k)
+        staged.append({f: torch.as_tensor(d[f], dtype=torch.float64, device=dev) for f in ("bed", "H", "beta", "B", "smb")})
+
+    def load(k, ph):
+        d = staged[k]
+        ph.set_geometry(d["bed"], d["H"])
+        ph.set_parameters(B=d["B"], beta=d["beta"], smb=d["smb"])
+
+    out = {"workload": f"{n_jobs} one-step jobs (seeds {TEST_SEED0 + 1}-{TEST_SEED0 + n_jobs}, {N512}x{N512}, "
+                       f"dx={DX512:.0f} m, {LEVELS512} levels, dt={dt:g} a, to 1e-8 of each job's cold residual, "
+                       f"tolerance schedule) through {n_slots} slots, refilled once {refill_min} are free"}
+    for mode in ("plain", "learned"):
+        if mode == "learned" and net is None:
+            break
+        slots = [member(TEST_SEED0 + 1, N512, dx=DX512, n_levels=LEVELS512) for _ in range(n_slots)]
+        ens = Ensemble(slots)
+        ens.set_schedule(**TS)
+        torch.cuda.synchronize()
+        st = stream_solve(ens, load, n_jobs, dt, tol=TOL, max_vcycles=MAX_VC, net=net if mode == "learned" else None,
+                          refill_min=refill_min)
+        vc = [j["vcycles"] for j in st.jobs]
+        out[mode] = {"wall_ms": round(st.wall_s * 1e3, 1), "ms_per_member_step": round(st.wall_s * 1e3 / n_jobs, 2),
+                     "vcycles_mean": round(float(np.mean(vc)), 3), "vcycles_max": int(max(vc)),
+                     "batched_vcycles": st.batched_vcycles, "slot_occupancy": round(st.occupancy, 3),
+                     "all_converged": bool(all(j["converged"] for j in st.jobs)),
+                     "fallbacks": int(sum(j["fallback"] for j in st.jobs))}
+        ens.close()
+        del slots, ens
+        torch.cuda.empty_cache()
+    if "learned" in out:
+        out["speedup_learned_vs_plain"] = round(out["plain"]["wall_ms"] / out["learned"]["wall_ms"], 3)
+    del staged
+    torch.cuda.empty_cache()
+    return out
+
+
 def reference_schedule_bench(name="cap_512", steps=3):
     """A parity workload on the reference's fixed step of 3 V-cycles: ms per
     step and per V-cycle and the relative residual reached.  cap_512 is the
@@ -713,6 +768,11 @@ def main():
         from glide_amd.warmstart import UNet
         net = UNet.load(WEIGHTS)
         print(json.dumps({f"unet_roofline_{n}": unet_roofline(net, n=n, reps=20) for n in (512, 1024)}))
+        return
+    if args.stream_only:
+        from glide_amd.warmstart import UNet
+        net = UNet.load(WEIGHTS)
+        print(json.dumps({f"refill_min_{k}": learned_stream_bench(net, refill_min=k) for k in (1, 4, 8, 16)}))
         return
     if args.only_1024:
         from glide_amd.warmstart import UNet

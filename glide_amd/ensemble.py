@@ -199,3 +199,128 @@ class Ensemble:
             for m in self.members:
                 m.grid.H_prev.copy_(m.grid.H)
         return st
+
+
+@dataclass
+class StreamStats:
+    """Outcome of a continuously batched run (stream_solve)."""
+    jobs: list = field(default_factory=list)   # per job: slot, vcycles, converged, used_net, fallback, times
+    wall_s: float = 0.0
+    batched_vcycles: int = 0                    # batched V-cycle launches over the whole run
+    occupancy: float = 0.0                      # mean fraction of slots solving a job per V-cycle
+
+
+def stream_solve(ens: Ensemble, load, n_jobs: int, dt: float, tol: float = 1e-8, max_vcycles: int = 100,
+                 net=None, on_done=None, refill_min: int = 1) -> StreamStats:
+    """Continuous batching: the ensemble's members are slots, and job k (one
+    implicit step of one domain, written into a slot by ``load(k, physics)``
+    through set_geometry / set_parameters) runs in the first free slot.  Each
+    job starts from u = v = 0 and H = H_prev, optionally seeded by ``net``
+    (the learned warm start with the fallback rule: the prediction is kept
+    only if its residual norm is at most the cold start's), and iterates in
+    the batched V-cycles until ||r|| <= tol ||r0_cold|| or max_vcycles; then
+    its slot takes the next job at once instead of idling until the slowest
+    member of a fixed batch finishes.  Every slot must have the size the
+    jobs need.  A job's states are those of solving it alone: the batched
+    V-cycle is the single-domain one, member by member, bit for bit.
+    ``on_done(k, physics)`` is called when job k ends, before its slot is
+    reused.  ``refill_min``: freed slots wait until at least this many are
+    free (or nothing else is running), so that new jobs start together; a
+    started job's first V-cycles run every local pass of the batch.  Every slot must have the grid size and dx of the jobs it takes
+    (the plan fixes both when it is built)."""
+    from .warmstart import IN_CH, N_OUT, apply_prediction, features, net_scales
+
+    S = ens.n
+    for m in ens.members:
+        for g in m.grids:
+            g.dt = float(dt)
+    ens.forward_setup(dt)
+    slot_job = [None] * S
+    vc = [0] * S
+    r0 = np.zeros(S)
+    st = StreamStats()
+    jobs = {}
+    next_job = 0
+    busy_sum = 0
+    t0 = time.perf_counter()
+
+    def refill(free):
+        nonlocal next_job
+        new = []
+        for m in free:
+            if next_job >= n_jobs:
+                break
+            k = next_job
+            next_job += 1
+            ph = ens.members[m]
+            load(k, ph)
+            g = ph.grid
+            g.u.zero_()
+            g.v.zero_()
+            g.H.copy_(g.H_prev)
+            _lib.call("glide_forward_setup", ph._levels(), ph.n_levels, float(dt), ph._stream())
+            slot_job[m], vc[m] = k, 0
+            jobs[k] = {"job": k, "slot": m, "t_start": time.perf_counter() - t0, "used_net": False,
+                       "fallback": False}
+            new.append(m)
+        if not new:
+            return
+        ens.set_enabled([slot_job[m] is not None for m in range(S)])
+        r = ens.residual_norms()
+        for m in new:
+            r0[m] = r[m]
+        if net is not None:
+            mem = [ens.members[m] for m in new]
+            g0 = mem[0].grid
+            X = torch.empty((len(new), g0.ny * g0.nx * IN_CH), dtype=torch.int16, device=g0.device)
+            for q, ph in enumerate(mem):
+                features(ph, out=X[q])
+            pred = net.forward(X, len(new), g0.ny, g0.nx)
+            n_px = g0.ny * g0.nx * N_OUT
+            for q, ph in enumerate(mem):
+                apply_prediction(ph, pred[q * n_px:(q + 1) * n_px], *net_scales(net))
+            rw = ens.residual_norms()
+            for m in new:
+                k = slot_job[m]
+                if rw[m] <= r0[m]:
+                    jobs[k]["used_net"] = True
+                else:
+                    g = ens.members[m].grid
+                    g.u.zero_()
+                    g.v.zero_()
+                    g.H.copy_(g.H_prev)
+                    jobs[k]["fallback"] = True
+
+    refill(list(range(S)))
+    pending = []
+    while any(j is not None for j in slot_job):
+        ens.vcycle()
+        st.batched_vcycles += 1
+        busy_sum += sum(j is not None for j in slot_job)
+        r = ens.residual_norms()
+        free = []
+        for m in range(S):
+            k = slot_job[m]
+            if k is None:
+                continue
+            vc[m] += 1
+            rel = r[m] / r0[m] if r0[m] > 0 else 0.0
+            if rel <= tol or vc[m] >= max_vcycles:
+                jobs[k].update(vcycles=vc[m], converged=bool(rel <= tol), final_rel_residual=float(rel),
+                               t_end=time.perf_counter() - t0)
+                if on_done is not None:
+                    on_done(k, ens.members[m])
+                slot_job[m] = None
+                free.append(m)
+        pending += free
+        if pending and (len(pending) >= refill_min or all(j is None for j in slot_job)):
+            refill(pending)
+            pending = []
+        if free:
+            ens.set_enabled([j is not None for j in slot_job])
+    torch.cuda.synchronize(ens.device)
+    st.wall_s = time.perf_counter() - t0
+    st.jobs = [jobs[k] for k in sorted(jobs)]
+    st.occupancy = busy_sum / (S * st.batched_vcycles) if st.batched_vcycles else 0.0
+    ens.set_enabled([True] * S)
+    return st

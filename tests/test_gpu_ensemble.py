@@ -14,7 +14,7 @@ if not torch.cuda.is_available():  # pragma: no cover
 from glide_amd import synthetic as S  # noqa: E402
 from glide_amd.autotune import tune  # noqa: E402
 from glide_amd.ensemble import (BK_RESIDUAL, BK_RESIDUAL_CHILD, BK_VANKA, BK_VANKA_LOCAL,  # noqa: E402
-                                Ensemble)
+                                Ensemble, stream_solve)
 from glide_amd.physics import IcePhysics  # noqa: E402
 
 # uneven sizes and level counts: 3, 2, 3, 2 and 1 level(s)
@@ -220,3 +220,45 @@ def test_full_size_members_one_vcycle():
         assert np.array_equal(state(solo), state(batch[k])), f"member {d['ny']}x{d['nx']}"
         del solo
     ens.close()
+
+
+def test_stream_solve_jobs_equal_their_solo_solves():
+    """Continuous batching: 7 one-step jobs (64^2 sheets, 3 levels, seeds
+    600-606) through 3 slots, each to 1e-8 of its cold residual.  Every job's
+    state when it ends equals the same job solved alone for the same number
+    of V-cycles, bit for bit, whatever the slot and whatever ran there
+    before; slots are refilled as soon as a job ends."""
+    from glide_amd.physics import TOLERANCE_SCHEDULE
+
+    n, dx, n_jobs = 64, 1500.0, 7
+    data = [S.ice_sheet(n, n, dx, 600 + k) for k in range(n_jobs)]
+
+    def fresh(d):
+        ph = IcePhysics(n, n, dx, n_levels=3)
+        ph.set_geometry(d["bed"], d["H"])
+        ph.set_parameters(B=d["B"], beta=d["beta"], smb=d["smb"])
+        ph.set_schedule(**TOLERANCE_SCHEDULE)
+        return ph
+
+    def load(k, ph):
+        d = data[k]
+        ph.set_geometry(d["bed"], d["H"])
+        ph.set_parameters(B=d["B"], beta=d["beta"], smb=d["smb"])
+
+    slots = [fresh(data[0]) for _ in range(3)]
+    ens = Ensemble(slots)
+    ens.set_schedule(**TOLERANCE_SCHEDULE)
+    done = {}
+    st = stream_solve(ens, load, n_jobs, dt=1.0, tol=1e-8, max_vcycles=80,
+                      on_done=lambda k, ph: done.__setitem__(k, state(ph)))
+    ens.close()
+    assert [j["job"] for j in st.jobs] == list(range(n_jobs)) and all(j["converged"] for j in st.jobs)
+    assert len({j["slot"] for j in st.jobs}) == 3
+    assert st.batched_vcycles < sum(j["vcycles"] for j in st.jobs)   # slots ran jobs side by side
+    print("jobs", [(j["job"], j["slot"], j["vcycles"]) for j in st.jobs], "batched V-cycles", st.batched_vcycles,
+          "occupancy", round(st.occupancy, 3), flush=True)
+    for j in st.jobs:
+        ph = fresh(data[j["job"]])
+        ph.forward(1.0, n_vcycles=j["vcycles"], update_geometry=False)
+        torch.cuda.synchronize()
+        assert np.array_equal(state(ph), done[j["job"]]), f"job {j['job']}"
