@@ -483,7 +483,6 @@ __global__ void k_forcing(size_t nc, double *__restrict__ fH, const double *__re
     if (k < nc) fH[k] = H_prev[k] / dt + smb[k];
 }
 
-// ||(r_u, r_v, FB(r_H, H - gamma))||^2: per-block partial sums (fixed tree order)
 // Count of non-finite entries of x (NaN or +-Inf) into *count.
 __global__ __launch_bounds__(256) void k_count_nonfinite(const double *__restrict__ x, size_t n,
                                                          unsigned long long *__restrict__ count)
@@ -498,6 +497,7 @@ __global__ __launch_bounds__(256) void k_count_nonfinite(const double *__restric
 
 __global__ void k_zero_u64(unsigned long long *__restrict__ x) { *x = 0ull; }
 
+// ||(r_u, r_v, FB(r_H, H - gamma))||^2: per-block partial sums (fixed tree order)
 __global__ __launch_bounds__(256) void k_norm_partial(int ny, int nx, const double *__restrict__ r,
                                                       const double *__restrict__ U,
                                                       const double *__restrict__ gamma, double *__restrict__ part)

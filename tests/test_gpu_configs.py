@@ -89,3 +89,57 @@ def test_warm_start_1024_converges_and_keeps_the_fallback_rule():
     assert stalled == 0, rows
     # both solves stop at 1e-8 of the same initial residual: the solutions agree to about that
     assert all(r[4] < 1e-6 for r in rows), rows
+
+
+def _lin(name, g, out, U, D):
+    import ctypes
+
+    from glide_amd import _lib
+    P = lambda t: ctypes.c_void_p(t.data_ptr())   # noqa: E731
+    _lib.call(name, ctypes.byref(g.geom()), ctypes.byref(g.cparams()), P(out), P(U), P(D), P(g.bed), P(g.B),
+              P(g.beta), P(g.mask), P(g.gamma), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+
+@pytest.mark.parametrize("n,levels,dx", [(512, 5, 1500.0), (1024, 6, 1000.0)])
+def test_full_size_linearisation_adjoint_and_norm(n, levels, dx):
+    """At the benchmark sizes, after one V-cycle of a test sheet:
+    <lam, J d> = <J^T lam, d> to 1e-12 between the staged JVP and VJP
+    (gl_derivatives off: exact transposes); the per-point forms give the
+    same bits as the 64 x 4 staged tiles with their XCD-chunked order; and
+    glide_residual_norm equals the norm of the GPU's own residual computed
+    with torch (sqrt(|r_u|^2 + |r_v|^2 + |FB(r_H, H - gamma)|^2)) to 1e-13."""
+    from glide_amd import _lib
+
+    ph = member(TEST_SEED0 + 70, n, dx=dx, n_levels=levels)
+    ph.set_schedule(**TOLERANCE_SCHEDULE)
+    ph.forward(1.0, n_vcycles=1, update_geometry=False)
+    g = ph.grid
+    gen = torch.Generator(device="cuda").manual_seed(n)
+    d = torch.randn(g.U.numel(), dtype=torch.float64, device="cuda", generator=gen)
+    lam = torch.randn(g.U.numel(), dtype=torch.float64, device="cuda", generator=gen)
+    jv, vt = torch.empty_like(d), torch.empty_like(d)
+    _lin("glide_jvp", g, jv, g.U, d)
+    _lin("glide_vjp", g, vt, g.U, lam)
+    a, b = float((lam * jv).sum()), float((vt * d).sum())
+    scale = float((lam.abs() * jv.abs()).sum())
+    print(f"{n}^2: <lam, Jd> = {a:.15e}, <J^T lam, d> = {b:.15e}, rel {abs(a - b) / scale:.2e}", flush=True)
+    assert abs(a - b) <= 1e-12 * scale
+    jv0, vt0 = torch.empty_like(d), torch.empty_like(d)
+    try:
+        _lib.call("glide_set_tile_height", 1, 0)
+        _lin("glide_jvp", g, jv0, g.U, d)
+        _lin("glide_vjp", g, vt0, g.U, lam)
+        torch.cuda.synchronize()
+    finally:
+        _lib.call("glide_set_tile_height", 1, 4)
+    assert torch.equal(jv, jv0) and torch.equal(vt, vt0)
+    got = ph.residual_norm()      # also leaves r = F(U) - f in g.r
+    nu, nv = g.nu, g.nv
+    rH = g.r[nu + nv:]
+    x = g.H.reshape(-1) - g.gamma.reshape(-1)
+    fb = rH + x - torch.sqrt(rH * rH + x * x)
+    ref = float(torch.sqrt((g.r[:nu] ** 2).sum() + (g.r[nu:nu + nv] ** 2).sum() + (fb ** 2).sum()))
+    print(f"{n}^2: residual norm {got:.15e}, torch {ref:.15e}", flush=True)
+    assert abs(got - ref) <= 1e-13 * ref
+    del ph
+    torch.cuda.empty_cache()
