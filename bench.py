@@ -850,7 +850,7 @@ def main():
     line["learned_batched"] = learned_batched_bench(net, [r["seed"] for r in rows])
     lb = line["learned_batched"].get("learned")
     if lb is not None and plain_ms:
-        # the north star's comparison: learned warm start + batching against plain single solves
+        # learned warm start + batching against plain single solves
         line["learned_batched_ms_per_member_step"] = lb["ms_per_member_step"]
         line["speedup_learned_batched_vs_plain"] = round(plain_ms / lb["ms_per_member_step"], 3)
     line["reference_schedule"] = reference_schedule_bench("cap_512")
