@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline solves and the kernel rooflines (for profiling them)")
     ap.add_argument("--cpu-report", action="store_true",
-                    help="only the CPU oracle: ms per V-cycle and per step of configs 0 and 1, one thread and all")
+                    help="only the CPU oracle: ms per V-cycle and per step of dome_128 and cap_512, one thread and all")
     return ap.parse_args()
 
 
@@ -237,7 +237,7 @@ def unet_roofline(net, n=512, reps=10):
 
 
 def cpu_report(threads_all: int):
-    """The CPU oracle (the reference has no CPU path) on configs 0 and 1:
+    """The CPU oracle (the reference has no CPU path) on dome_128 and cap_512:
     ms per V-cycle and per 3-V-cycle step on one thread and on all the given
     threads, with the host's CPU model."""
     from glide_amd import synthetic as S

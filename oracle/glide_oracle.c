@@ -3,7 +3,7 @@
  *
  * FP64 CPU restatement of GLIDE's SSA / FASCD solver.  Only tests/, the
  * smoke() check and bench.py's cpu_baseline leg may load this library, and
- * only as the checker (or as the timed CPU baseline).  The product path
+ * only as the reference results (or as the timed CPU baseline).  The product path
  * (glide_amd/) never links, imports or calls it.
  *
  * Every function follows the reference's algorithm in FP64 with the

@@ -3,7 +3,7 @@
 ctypes wrapper around ``oracle/_build/liboracle.so``, the FP64 CPU
 restatement of GLIDE's SSA/FASCD solver (see ``glide_oracle.c``).  Only the
 test suite, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
-load this module, and only as the checker / timed CPU baseline; the product
+load this module, and only as the reference results / timed CPU baseline; the product
 package ``glide_amd`` never imports it.
 """
 from __future__ import annotations
