@@ -363,10 +363,14 @@ __device__ __forceinline__ void vanka_tile(const GlideGeom &g, const Params &p, 
     const double *u = U, *v = U + nu, *H = U + nu + nv;
     const int i0 = by * TY, j0 = bx * TX;
     const int i = i0 + threadIdx.y, j = j0 + threadIdx.x;
-    bool mine = (i < ny && j < nx);
+    const bool mine = (i < ny && j < nx);
+    // A local sweep smooths every cell of a tile that holds an active cell:
+    // the apply pass reads only the active cells' updates, so the idle
+    // cells' chains change nothing, and a wave with part of its lanes idle
+    // ran its Newton chain up to 2.3x slower than a full wave (measured on
+    // the 64^2 to 512^2 levels).
     if (LOCAL) {
-        mine = mine && blk[(i / RB) * nbx + j / RB];
-        if (!__syncthreads_or(mine)) return;   // uniform: the whole tile is idle
+        if (!__syncthreads_or(mine && blk[(i / RB) * nbx + j / RB])) return;   // uniform: the whole tile is idle
     }
     fill_eta_tile<TX, TY>(eta_s, i0, j0, u, v, B, p, g.dx, ny, nx);
     __syncthreads();
