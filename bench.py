@@ -489,9 +489,10 @@ def level_sweep_roofline(ph, level: int, n_sweeps: int, reps: int = 3):
     ms = e0.elapsed_time(e1) / (reps * n_sweeps)
     nc = g.ny * g.nx
     tf = sweep_flops(nc) / (ms * 1e-3) / 1e12
-    return {"kernel": f"k_vanka<false, 1> + k_vanka_apply<false> (full sweep of the {g.ny}x{g.nx} level)",
+    return {"kernel": f"k_vanka_halo<false> (full sweep of the {g.ny}x{g.nx} level, apply pass folded in)",
             "bound": "valu", "note": "FP64 vector arithmetic (no MFMA); peak = MI355X FP64 78.6 TFLOP/s; one "
-            "lane per cell, latency of the per-cell Newton chain plus two launches per sweep",
+            "lane per cell (each workgroup also runs the ring around its 62 x 2 cells), latency of the per-cell "
+            "Newton chain plus one launch per sweep",
             "achieved": round(tf, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": round(tf / PEAK_FP64_TFLOPS, 5), "traffic": None, "ms_per_launch": round(ms, 5),
             "work": f"{nc} cells x (eta + 10 Newton x {FLOPS_NEWTON_ITER} flop) = {sweep_flops(nc)} flop per sweep"}

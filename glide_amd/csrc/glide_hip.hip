@@ -65,11 +65,15 @@ inline hipStream_t S(void *s) { return (hipStream_t)s; }
 std::atomic<int> g_wg_per_cu[2] = {{64}, {8}};
 // Levels with at most g_small_max_cells cells sweep with the small-level
 // kernel: variant 0 = k_vanka compiled for one wave per SIMD (the whole
-// register file), 1 = the row split (k_vanka_rows); set by
-// glide_set_small_levels.  Measured (sweep + apply, 10 Newton iterations):
-// 32^2 22.5 / 23.2 us, 64^2 23.6 / 24.0, 128^2 23.4 / 24.1 (variant 0 / 1).
+// register file), 1 = the row split (k_vanka_rows), 2 = k_vanka_halo (the
+// apply pass folded into the sweep, the state alternating between U and the
+// Vanka scratch); set by glide_set_small_levels.  Measured (one sweep with
+// its apply, 10 Newton iterations, HIP events over 10 sweeps): 32^2 22.2 /
+// 19.9 us, 64^2 23.4 / 20.5, 128^2 23.7-24.6 / 20.7-21.1 (variant 0 / 2);
+// the row split measured 23.2, 24.0 and 24.1 against 22.5, 23.6 and 23.4
+// for variant 0 in an earlier build.
 std::atomic<long> g_small_max_cells{128 * 128};
-std::atomic<int> g_small_variant{0};
+std::atomic<int> g_small_variant{2};
 // Tile heights (64 x TY points) of the single-level residual (4, 8, 16; 0 =
 // by level size) and of the JVP / VJP (0 = the per-point form, 4, 6, 8); set
 // by glide_set_tile_height.  Launch shapes only: results are the same bits.
@@ -158,6 +162,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         vanka_tile<TX, TY, LOCAL>(g, p, U, f, bed, B, beta, gamma, blk, nbx, n_newton, cd, t % tx, t / tx, eta_s);
     });
 }
+
+// Small-level sweep with the apply pass folded in (vanka_halo_tile): U -> Uo,
+// one workgroup per 62 x 2 owned cells, one wave per SIMD.
+template <bool LOCAL>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1))) void k_vanka_halo(
+    GlideGeom g, Params p, const double *__restrict__ U, double *__restrict__ Uo, const double *__restrict__ f,
+    const double *__restrict__ bed, const double *__restrict__ B, const double *__restrict__ beta,
+    const double *__restrict__ gamma, const unsigned char *__restrict__ blk, int nbx, int n_newton)
+{
+    __shared__ double eta_s[TY + 2][TX + 2];
+    __shared__ double cd_s[5][TY][TX];
+    vanka_halo_tile<TX, TY, LOCAL>(g, p, U, Uo, f, bed, B, beta, gamma, blk, nbx, n_newton, blockIdx.x, blockIdx.y,
+                                   eta_s, cd_s);
+}
+dim3 halo_grid(int ny, int nx) { return dim3(cdiv(nx, TX - 2), cdiv(ny, TY - 2)); }
 
 
 // ---------------------------------------------------------------------------
@@ -1077,6 +1096,38 @@ int launch_sweeps(const GlideGeom &g, const Params &p, double *U, const double *
     const bool small = (long)g.ny * g.nx <= g_small_max_cells;
     const bool rows = small && g_small_variant == 1;
     const dim3 rgrid(cdiv(g.nx, 64), g.ny), rblock(64, ROW_WAVES);
+    if (small && g_small_variant == 2) {
+        // The sweeps alternate the state between U and the scratch (5 nc >=
+        // nu + nv + nc doubles); an odd count runs its first sweep in place
+        // with the apply pass, so that the last sweep writes U.
+        int s = 0;
+        if (n_sweeps & 1) {
+            if (blk) {
+                hipLaunchKernelGGL((k_vanka<true, 1>), vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed,
+                                   B, beta, gamma, blk, nbx, n_newton, scratch);
+                hipLaunchKernelGGL(k_vanka_apply<true>, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g.ny, g.nx, U,
+                                   scratch, blk, nbx);
+            } else {
+                hipLaunchKernelGGL((k_vanka<false, 1>), vanka_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, U, f, bed,
+                                   B, beta, gamma, blk, nbx, n_newton, scratch);
+                hipLaunchKernelGGL(k_vanka_apply<false>, point_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g.ny, g.nx, U,
+                                   scratch, blk, nbx);
+            }
+            s = 1;
+        }
+        for (; s < n_sweeps; s++) {
+            const bool to_u = ((n_sweeps - 1 - s) & 1) == 0;
+            const double *src = to_u ? scratch : U;
+            double *dst = to_u ? U : scratch;
+            if (blk)
+                hipLaunchKernelGGL(k_vanka_halo<true>, halo_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, src, dst, f,
+                                   bed, B, beta, gamma, blk, nbx, n_newton);
+            else
+                hipLaunchKernelGGL(k_vanka_halo<false>, halo_grid(g.ny, g.nx), dim3(TX, TY), 0, st, g, p, src, dst,
+                                   f, bed, B, beta, gamma, blk, nbx, n_newton);
+        }
+        return check_launch("vanka sweep");
+    }
     for (int s = 0; s < n_sweeps; s++) {
         if (rows) {
             if (blk) {
@@ -1491,7 +1542,8 @@ int glide_vcycle_work_counts(GlideVcycleWork *w, long *run, long *skipped, int r
 int glide_set_small_levels(long max_cells, int variant)
 {
     if (max_cells < 0) return fail(GLIDE_E_ARG, "max_cells must be >= 0");
-    if (variant != 0 && variant != 1) return fail(GLIDE_E_ARG, "small-level variant must be 0 (per-cell) or 1 (row split)");
+    if (variant < 0 || variant > 2)
+        return fail(GLIDE_E_ARG, "small-level variant must be 0 (per-cell), 1 (row split) or 2 (apply folded in)");
     g_small_max_cells = max_cells;
     g_small_variant = variant;
     return GLIDE_OK;

@@ -310,11 +310,11 @@ __device__ __forceinline__ CellNbhd load_nbhd(int i, int j, int ny, int nx, size
 // arithmetic, and so every bit of the result, is unchanged.
 // ---------------------------------------------------------------------------
 template <bool INTERIOR, class EtaFn>
-__device__ __forceinline__ void vanka_cell(const GlideGeom &g, const Params &p, const double *__restrict__ u,
-                                           const double *__restrict__ v, const double *__restrict__ H,
-                                           const double *__restrict__ f, const double *__restrict__ bed,
-                                           const double *__restrict__ beta, const double *__restrict__ gamma,
-                                           int n_newton, double *__restrict__ cd, int i, int j, const EtaFn &E)
+__device__ __forceinline__ void vanka_cell_deltas(const GlideGeom &g, const Params &p, const double *__restrict__ u,
+                                                  const double *__restrict__ v, const double *__restrict__ H,
+                                                  const double *__restrict__ f, const double *__restrict__ bed,
+                                                  const double *__restrict__ beta, const double *__restrict__ gamma,
+                                                  int n_newton, int i, int j, const EtaFn &E, double d5[5])
 {
     const int ny = g.ny, nx = g.nx;
     const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
@@ -337,12 +337,28 @@ __device__ __forceinline__ void vanka_cell(const GlideGeom &g, const Params &p, 
         H_c -= 0.25 * d[4];
         H_c = fmax(H_c, th);
     }
-    const size_t ci = (size_t)i * nx + j;
-    cd[ci] = 0.5 * (u_l - u_l0);
-    cd[nc + ci] = 0.5 * (u_r - u_r0);
-    cd[2 * nc + ci] = 0.5 * (v_t - v_t0);
-    cd[3 * nc + ci] = 0.5 * (v_b - v_b0);
-    cd[4 * nc + ci] = H_c - H_c0;
+    d5[0] = 0.5 * (u_l - u_l0);
+    d5[1] = 0.5 * (u_r - u_r0);
+    d5[2] = 0.5 * (v_t - v_t0);
+    d5[3] = 0.5 * (v_b - v_b0);
+    d5[4] = H_c - H_c0;
+}
+
+template <bool INTERIOR, class EtaFn>
+__device__ __forceinline__ void vanka_cell(const GlideGeom &g, const Params &p, const double *__restrict__ u,
+                                           const double *__restrict__ v, const double *__restrict__ H,
+                                           const double *__restrict__ f, const double *__restrict__ bed,
+                                           const double *__restrict__ beta, const double *__restrict__ gamma,
+                                           int n_newton, double *__restrict__ cd, int i, int j, const EtaFn &E)
+{
+    double d5[5];
+    vanka_cell_deltas<INTERIOR>(g, p, u, v, H, f, bed, beta, gamma, n_newton, i, j, E, d5);
+    const size_t nc = n_c(g.ny, g.nx), ci = (size_t)i * g.nx + j;
+    cd[ci] = d5[0];
+    cd[nc + ci] = d5[1];
+    cd[2 * nc + ci] = d5[2];
+    cd[3 * nc + ci] = d5[3];
+    cd[4 * nc + ci] = d5[4];
 }
 
 // ---------------------------------------------------------------------------
@@ -411,6 +427,77 @@ __device__ __forceinline__ void apply_tile(int ny, int nx, double *__restrict__ 
         size_t ci = (size_t)i * nx + j;
         double d = act(i, j) ? cd[4 * nc + ci] : 0.0;
         U[nu + nv + ci] += 0.5 * d;
+    }
+}
+
+// One Vanka sweep with its apply pass folded in, from the state U into Uo
+// (another buffer).  The workgroup's TX x TY threads run the cells
+// [i0-1, i0+TY-2] x [j0-1, j0+TX-2]: the (TX-2) x (TY-2) cells it owns,
+// (i0, j0) = (by (TY-2), bx (TX-2)), and the ring around them, whose
+// updates the owned faces also receive.  The per-cell updates meet in LDS
+// and every point the workgroup owns gets U + 0.5 d with d summed as the
+// apply pass sums it: the u faces and H of its cells, their top v faces,
+// and the level's last face column and row where its cells reach them.
+// Each point of the level is owned once, so Uo is written whole.  Threads
+// whose cell lies outside the level run the nearest cell of the level and
+// drop the result (a wave with idle lanes ran its chain slower).  With
+// LOCAL, idle cells' updates are not read (as in the apply pass), and a
+// workgroup whose cells are all idle runs no chain.  Same bits as vanka_tile
+// followed by apply_tile.  The grid is cdiv(nx, TX-2) x cdiv(ny, TY-2).
+template <int TX, int TY, bool LOCAL>
+__device__ __forceinline__ void vanka_halo_tile(const GlideGeom &g, const Params &p, const double *__restrict__ U,
+                                                double *__restrict__ Uo, const double *__restrict__ f,
+                                                const double *__restrict__ bed, const double *__restrict__ B,
+                                                const double *__restrict__ beta, const double *__restrict__ gamma,
+                                                const unsigned char *__restrict__ blk, int nbx, int n_newton, int bx,
+                                                int by, double (*eta_s)[TX + 2], double (*cd_s)[TY][TX])
+{
+    const int ny = g.ny, nx = g.nx;
+    const size_t nu = n_u(ny, nx), nv = n_v(ny, nx);
+    const double *u = U, *v = U + nu, *H = U + nu + nv;
+    const int i0 = by * (TY - 2), j0 = bx * (TX - 2);   // owned cells [i0, ie) x [j0, je)
+    const int ie = min(i0 + TY - 2, ny), je = min(j0 + TX - 2, nx);
+    const int a = threadIdx.y, b = threadIdx.x;
+    const int i = i0 - 1 + a, j = j0 - 1 + b;
+    const bool in = i >= 0 && i < ny && j >= 0 && j < nx;
+    auto act = [&](int ci, int cj) { return !LOCAL || blk[(ci / RB) * nbx + cj / RB] != 0; };
+    bool run = true;
+    if (LOCAL) run = __syncthreads_or(in && act(i, j));   // uniform
+    if (run) {
+        fill_eta_tile<TX, TY>(eta_s, i0 - 1, j0 - 1, u, v, B, p, g.dx, ny, nx);
+        __syncthreads();
+        const int ic = min(max(i, 0), ny - 1), jc = min(max(j, 0), nx - 1);
+        auto E = [&](int ci, int cj) { return eta_s[ci - i0 + 2][cj - j0 + 2]; };
+        double d5[5];
+        // every run cell in [1, ny-2] x [1, nx-2]: the boundary tests resolved at compile time
+        if (i0 >= 2 && i0 + TY <= ny && j0 >= 2 && j0 + TX <= nx)
+            vanka_cell_deltas<true>(g, p, u, v, H, f, bed, beta, gamma, n_newton, ic, jc, E, d5);
+        else
+            vanka_cell_deltas<false>(g, p, u, v, H, f, bed, beta, gamma, n_newton, ic, jc, E, d5);
+#pragma unroll
+        for (int q = 0; q < 5; q++) cd_s[q][a][b] = d5[q];
+        __syncthreads();
+    }
+    auto cdv = [&](int q, int ci, int cj) { return cd_s[q][ci - i0 + 1][cj - j0 + 1]; };
+    double *uo = Uo, *vo = Uo + nu, *Ho = Uo + nu + nv;
+    if (i >= i0 && i < ie && j >= j0 && (j < je || j == nx)) {
+        double d = 0.0;
+        if (j > 0 && act(i, j - 1)) d = d + cdv(1, i, j - 1);
+        if (j < nx && act(i, j)) d = d + cdv(0, i, j);
+        const size_t k = (size_t)i * (nx + 1) + j;
+        uo[k] = u[k] + 0.5 * d;
+    }
+    if (j >= j0 && j < je && i >= i0 && (i < ie || i == ny)) {
+        double d = 0.0;
+        if (i > 0 && act(i - 1, j)) d = d + cdv(3, i - 1, j);
+        if (i < ny && act(i, j)) d = d + cdv(2, i, j);
+        const size_t k = (size_t)i * nx + j;
+        vo[k] = v[k] + 0.5 * d;
+    }
+    if (i >= i0 && i < ie && j >= j0 && j < je) {
+        const size_t k = (size_t)i * nx + j;
+        const double d = act(i, j) ? cdv(4, i, j) : 0.0;
+        Ho[k] = H[k] + 0.5 * d;
     }
 }
 

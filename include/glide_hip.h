@@ -64,7 +64,8 @@ typedef struct GlideLevel {
     double *U, *f, *r, *w, *z;                   /* n_total each */
     double *Lambda, *Lambda_out, *f_adj, *r_adj; /* n_total each (adjoint) */
     double *H_prev, *bed, *B, *beta, *smb, *gamma, *chi, *mask; /* nc each */
-    double *scratch;      /* 5 * nc doubles (Vanka per-cell updates) */
+    double *scratch;      /* 5 * nc doubles (Vanka per-cell updates; the small-level
+                             sweeps' alternate state) */
     unsigned char *active;/* (ny/14+1)*(nx/14+1) bytes: local-smoother blocks */
 } GlideLevel;
 
@@ -357,10 +358,13 @@ int glide_set_tile_height(int kernel, int ty);
 
 /* Vanka kernel of the small levels, whose sweeps are bound by the latency
  * of one wave's Newton chain: levels with at most max_cells cells (default
- * 128 x 128) sweep with `variant` 0 (default) = one lane per cell with the
- * whole register file, or 1 = row split (five waves per 64 cells, each assembling
- * one row of the local block); larger levels sweep one lane per cell at two
- * waves per SIMD.  A launch shape only: results are bitwise the same.
+ * 128 x 128) sweep with `variant` 0 = one lane per cell with the whole
+ * register file, 1 = row split (five waves per 64 cells, each assembling one
+ * row of the local block), or 2 (default) = variant 0 with the apply pass
+ * folded into the sweep (each workgroup also runs the ring of cells around
+ * the 62 x 2 cells it owns; the state alternates between U and the scratch);
+ * larger levels sweep one lane per cell at two waves per SIMD.  A launch
+ * shape only: results are bitwise the same.
  * Process-wide (atomic); no reference counterpart. */
 int glide_set_small_levels(long max_cells, int variant);
 

@@ -116,9 +116,10 @@ def test_residual_tile_heights_bitwise(ty, shape):
 @pytest.fixture
 def small_levels():
     """Set the small-level Vanka variant for one test; restores the library
-    default (levels up to 128 x 128 cells, one lane per cell) afterwards."""
+    default (levels up to 128 x 128 cells, one lane per cell with the apply
+    pass folded in) afterwards."""
     yield lambda cells, variant: _lib.call("glide_set_small_levels", cells, variant)
-    _lib.call("glide_set_small_levels", 128 * 128, 0)
+    _lib.call("glide_set_small_levels", 128 * 128, 2)
 
 
 # Every shape sweeps with each kernel: the row split and the full-register
@@ -142,9 +143,32 @@ def test_vanka_sweep_bitwise(shape, variant, small_levels):
     assert np.array_equal(out, ref), np.abs(out - ref).max()
 
 
-@pytest.mark.parametrize("variant", ["rows", "small", "large"])
+# The folded-apply small-level kernel alternates the state between U and the
+# scratch, so it runs for two or more sweeps: odd counts start with one
+# in-place sweep.  Shapes: owned column blocks of 62 that end exactly at the
+# last column (124) or one short of it, odd row counts, interior tiles.
+@pytest.mark.parametrize("n_sweeps", [2, 3])
+@pytest.mark.parametrize("shape", [(9, 12), (7, 124), (40, 123), (131, 150)])
+def test_vanka_sweeps_folded_apply_bitwise(shape, n_sweeps, small_levels):
+    ny, nx = shape
+    small_levels(1 << 20, 2)
+    s = random_state(ny=ny, nx=nx, seed=5 * ny + nx, vel=100.0)
+    prm = O.params()
+    ref = s["U"]
+    for _ in range(n_sweeps):
+        ref = O.vanka_sweep(ny, nx, s["dx"], s["dt"], prm, ref, s["f"], s["bed"], s["B"], s["beta"], s["gamma"])
+    g = _lib.GlideGeom(ny, nx, s["dx"], s["dt"])
+    U = T(s["U"])
+    scratch = torch.zeros(5 * ny * nx, dtype=torch.float64, device=DEV)
+    _lib.call("glide_vanka_sweep", ctypes.byref(g), ctypes.byref(cparams(prm)), P(U), P(T(s["f"])), P(T(s["bed"])),
+              P(T(s["B"])), P(T(s["beta"])), P(T(s["gamma"])), None, n_sweeps, 10, P(scratch), stream())
+    out = U.cpu().numpy()
+    assert np.array_equal(out, ref), np.abs(out - ref).max()
+
+
+@pytest.mark.parametrize("variant", ["rows", "small", "large", "folded"])
 def test_local_activation_and_local_sweep_bitwise(variant, small_levels):
-    small_levels(*{"rows": (1 << 20, 1), "small": (1 << 20, 0), "large": (0, 0)}[variant])
+    small_levels(*{"rows": (1 << 20, 1), "small": (1 << 20, 0), "large": (0, 0), "folded": (1 << 20, 2)}[variant])
     ny, nx = 48, 60
     s = random_state(ny=ny, nx=nx, seed=77, vel=50.0)
     prm = O.params()
@@ -165,12 +189,16 @@ def test_local_activation_and_local_sweep_bitwise(variant, small_levels):
     cells = np.array([[b[i // 14, j // 14] for j in range(nx)] for i in range(ny)])
     assert np.array_equal(cells.astype(np.uint8), act_cells)
     assert 0 < cells.sum() < ny * nx
-    ref = O.vanka_sweep(ny, nx, s["dx"], s["dt"], prm, s["U"], s["f"], s["bed"], s["B"], s["beta"], s["gamma"],
-                        active=act_cells)
+    # the folded-apply kernel runs from the second sweep on
+    n_sweeps = 2 if variant == "folded" else 1
+    ref = s["U"]
+    for _ in range(n_sweeps):
+        ref = O.vanka_sweep(ny, nx, s["dx"], s["dt"], prm, ref, s["f"], s["bed"], s["B"], s["beta"], s["gamma"],
+                            active=act_cells)
     U = T(s["U"])
     scratch = torch.zeros(5 * ny * nx, dtype=torch.float64, device=DEV)
     _lib.call("glide_vanka_sweep", ctypes.byref(g), ctypes.byref(cparams(prm)), P(U), P(T(s["f"])), P(T(s["bed"])),
-              P(T(s["B"])), P(T(s["beta"])), P(T(s["gamma"])), P(blk), 1, 10, P(scratch), stream())
+              P(T(s["B"])), P(T(s["beta"])), P(T(s["gamma"])), P(blk), n_sweeps, 10, P(scratch), stream())
     assert np.array_equal(U.cpu().numpy(), ref)
 
 
