@@ -143,7 +143,8 @@ int glide_vanka_sweep(const GlideGeom *g, const GlideParams *p, double *U, const
 /* glide_vanka_sweep without the scratch argument, for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc): no counterpart in the reference.  The per-cell
- * updates go to a static device array of 5 x 2^20 doubles (40 MiB) that
+ * updates (and the small levels' alternate state, see
+ * glide_set_small_levels) go to a static device array of 5 x 2^20 doubles (40 MiB) that
  * each GPU the library runs on holds until the process ends, so levels up
  * to 2^20 cells (1024 x 1024) are accepted and larger ones fail with
  * GLIDE_E_SIZE.  Nothing is allocated; the array is shared by every call on
