@@ -455,9 +455,11 @@ def reference_settings_bench(seeds, dt=DT, max_vcycles=MAX_VC):
 
 
 def level_sweep_roofline(ph, level: int, n_sweeps: int, reps: int = 3):
-    """n_sweeps full Vanka sweeps (each a sweep launch and an apply launch)
-    on one level of the hierarchy, HIP-event timed on the solver's stream;
-    priced at the counted per-cell flops against the FP64 vector peak."""
+    """n_sweeps full Vanka sweeps on one level of the hierarchy (on levels of
+    at most 128^2 cells one launch each, the apply pass folded in; an even
+    n_sweeps keeps every sweep on that kernel), HIP-event timed on the
+    solver's stream; priced at the counted per-cell flops against the FP64
+    vector peak."""
     import ctypes
 
     import torch
