@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--no-1024", action="store_true", help="skip the 1024^2 warm-start solves")
     ap.add_argument("--ensemble-only", action="store_true", help="only the 64-member batched ensemble")
     ap.add_argument("--only-1024", action="store_true", help="only the 1024^2 warm-start solves")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the 16-member batched headline solves, plain and learned (for profiling them)")
     ap.add_argument("--unet-only", action="store_true", help="only the warm-start network's inference rooflines")
     ap.add_argument("--stream-only", action="store_true",
                     help="only the continuously batched solves (slots refilled at 1, 4, 8 and 16 free)")
@@ -780,6 +782,12 @@ def main():
     if args.only_1024:
         from glide_amd.warmstart import UNet
         print(json.dumps(learned_1024_bench(UNet.load(WEIGHTS))))
+        return
+    if args.batched_only:
+        from glide_amd.train_warmstart import TEST_SEED0
+        from glide_amd.warmstart import UNet
+        seeds = [TEST_SEED0 + args.warmup + k for k in range(args.steps)]
+        print(json.dumps(learned_batched_bench(UNet.load(WEIGHTS), seeds)))
         return
     if args.kernels_only:
         from glide_amd import synthetic as S
