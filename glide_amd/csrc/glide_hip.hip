@@ -396,6 +396,37 @@ __global__ void k_restrict_max(const double *__restrict__ hf, double *__restrict
     double b = fmax(hf[(2 * J + 1) * nxf + 2 * I], hf[(2 * J + 1) * nxf + 2 * I + 1]);
     hc[idx] = fmax(a, b);
 }
+// The three restrictions of a state vector [u | v | H] in one launch (the
+// arithmetic of k_restrict_u, k_restrict_v and k_restrict_cell per point).
+__global__ void k_restrict_state(const double *__restrict__ src, double *__restrict__ dst, double *__restrict__ dst2,
+                                 int nyc, int nxc, double bw)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nuc = n_u(nyc, nxc), nvc = n_v(nyc, nxc), ncc = n_c(nyc, nxc);
+    const size_t nxf = 2 * (size_t)nxc, nuf = n_u(2 * nyc, 2 * nxc), nvf = n_v(2 * nyc, 2 * nxc);
+    double val;
+    if (idx < nuc) {
+        const int J = (int)(idx / (nxc + 1)), I = (int)(idx - (size_t)J * (nxc + 1));
+        val = 0.5 * (src[(size_t)(2 * J) * (nxf + 1) + 2 * I] + src[(size_t)(2 * J + 1) * (nxf + 1) + 2 * I]);
+        if (I == 0 || I == nxc) val *= bw;
+    } else if (idx < nuc + nvc) {
+        const size_t k = idx - nuc;
+        const int J = (int)(k / nxc), I = (int)(k - (size_t)J * nxc);
+        const double *vf = src + nuf;
+        val = 0.5 * (vf[(size_t)(2 * J) * nxf + 2 * I] + vf[(size_t)(2 * J) * nxf + 2 * I + 1]);
+        if (J == 0 || J == nyc) val *= bw;
+    } else if (idx < nuc + nvc + ncc) {
+        const size_t k = idx - nuc - nvc;
+        const int J = (int)(k / nxc), I = (int)(k - (size_t)J * nxc);
+        const double *hf = src + nuf + nvf;
+        val = 0.25 * (hf[(2 * J) * nxf + 2 * I] + hf[(2 * J) * nxf + 2 * I + 1] + hf[(2 * J + 1) * nxf + 2 * I] +
+                      hf[(2 * J + 1) * nxf + 2 * I + 1]);
+    } else {
+        return;
+    }
+    dst[idx] = val;
+    if (dst2) dst2[idx] = val;
+}
 __global__ void k_prolong_inject(const double *__restrict__ hc, double *__restrict__ hf, int nyf, int nxf, double sign)
 {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -465,10 +496,29 @@ __global__ void k_gamma(int ny, int nx, double *__restrict__ gamma, const double
     }
     gamma[idx] = wH[idx] + ph;
 }
+// k_restrict_max into chi_c followed by the pre-smoothing k_gamma, in one
+// launch: each fine cell forms its parent's max itself (the same fmax
+// order), and the cell at the parent's top-left corner stores it.
+__global__ void k_gamma_pre(int ny, int nx, double *__restrict__ gamma, const double *__restrict__ wH,
+                            const double *__restrict__ chi, double *__restrict__ chi_c)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ny * nx) return;
+    const int j = idx / nx, i = idx - j * nx, J = j / 2, I = i / 2;
+    const size_t f0 = (size_t)(2 * J) * nx + 2 * I, f1 = f0 + nx;
+    const double a = fmax(chi[f0], chi[f0 + 1]);
+    const double b = fmax(chi[f1], chi[f1 + 1]);
+    const double c = fmax(a, b);
+    if (j == 2 * J && i == 2 * I) chi_c[(size_t)J * (nx / 2) + I] = c;
+    const double inj = -1.0 * c;
+    gamma[idx] = wH[idx] + (inj + chi[idx]);
+}
 // U = w + (P(U_c - w_c) + (U - w)): bilinear for u, v and injection for H
-// (glide/solver.py:131-140)
+// (glide/solver.py:131-140); with gamma, the cells also take the
+// post-correction k_gamma, gamma = w_H + chi (it reads no U).
 __global__ void k_fas_correct(int ny, int nx, double *__restrict__ U, const double *__restrict__ w,
-                              const double *__restrict__ Uc, const double *__restrict__ wc)
+                              const double *__restrict__ Uc, const double *__restrict__ wc,
+                              double *__restrict__ gamma, const double *__restrict__ chi)
 {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
@@ -488,6 +538,7 @@ __global__ void k_fas_correct(int ny, int nx, double *__restrict__ U, const doub
         int j = (int)(kk / nx), i = (int)(kk - (size_t)j * nx);
         size_t kc = nuc + nvc + (size_t)(j / 2) * nxc + i / 2;
         z = Uc[kc] - wc[kc];
+        if (gamma) gamma[kk] = w[k] + chi[kk];
     }
     double y = U[k] - w[k];
     z += y;
@@ -1231,16 +1282,14 @@ int read_flag(GlideVcycleWork *w, int k, bool *any)
     return GLIDE_OK;
 }
 
+// F is the 2x refinement of C (check_levels)
 int restrict_state(const GlideLevel &F, const GlideLevel &C, const double *src, double *dst, double *dst2,
                    hipStream_t st, double bw = 1.0)
 {
-    const int nyc = C.g.ny, nxc = C.g.nx, nyf = F.g.ny, nxf = F.g.nx;
-    const size_t nuf = n_u(nyf, nxf), nvf = n_v(nyf, nxf), nuc = n_u(nyc, nxc), nvc = n_v(nyc, nxc);
-    hipLaunchKernelGGL(k_restrict_u, dim3(blocks1d(nuc)), dim3(256), 0, st, src, dst, dst2, nyc, nxc, bw);
-    hipLaunchKernelGGL(k_restrict_v, dim3(blocks1d(nvc)), dim3(256), 0, st, src + nuf, dst + nuc,
-                       dst2 ? dst2 + nuc : nullptr, nyc, nxc, bw);
-    hipLaunchKernelGGL(k_restrict_cell, dim3(blocks1d(n_c(nyc, nxc))), dim3(256), 0, st, src + nuf + nvf,
-                       dst + nuc + nvc, dst2 ? dst2 + nuc + nvc : nullptr, nyc, nxc);
+    (void)F;
+    const int nyc = C.g.ny, nxc = C.g.nx;
+    hipLaunchKernelGGL(k_restrict_state, dim3(blocks1d(n_t(nyc, nxc))), dim3(256), 0, st, src, dst, dst2, nyc, nxc,
+                       bw);
     return check_launch("restrict state");
 }
 
@@ -1309,8 +1358,7 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
     const GlideLevel &C = L[k + 1];
     const int nyc = C.g.ny, nxc = C.g.nx;
     const Params pc = to_params(C.p);
-    hipLaunchKernelGGL(k_restrict_max, dim3(blocks1d(n_c(nyc, nxc))), dim3(256), 0, st, A.chi, C.chi, nyc, nxc);
-    hipLaunchKernelGGL(k_gamma, dim3(blocks1d(nc)), dim3(256), 0, st, ny, nx, A.gamma, wH, A.chi, C.chi);
+    hipLaunchKernelGGL(k_gamma_pre, dim3(blocks1d(nc)), dim3(256), 0, st, ny, nx, A.gamma, wH, A.chi, C.chi);
     TRY(check_launch("pre gamma"));
     TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, nullptr, sc.pre, sc.n_newton, A.scratch, st));
     TRY(fill(A.gamma, nc, thklim, st));
@@ -1320,8 +1368,8 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
     // f_c = F_c(U_c) - r_c
     TRY(launch_residual(C.g, pc, C.f, C.U, nullptr, C.r, C.bed, C.B, C.beta, C.mask, C.gamma, st));
     TRY(fascd(L, n, k + 1, false, thklim, sc, work, st));
-    hipLaunchKernelGGL(k_fas_correct, dim3(blocks1d(nt)), dim3(256), 0, st, ny, nx, A.U, A.w, C.U, C.w);
-    hipLaunchKernelGGL(k_gamma, dim3(blocks1d(nc)), dim3(256), 0, st, ny, nx, A.gamma, wH, A.chi, nullptr);
+    hipLaunchKernelGGL(k_fas_correct, dim3(blocks1d(nt)), dim3(256), 0, st, ny, nx, A.U, A.w, C.U, C.w, A.gamma,
+                       A.chi);
     TRY(check_launch("fas correct"));
     TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, nullptr, sc.post, sc.n_newton, A.scratch, st));
     // A pass whose activation marks no block leaves U unchanged (its apply
