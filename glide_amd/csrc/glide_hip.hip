@@ -10,6 +10,7 @@
 // so no atomics are needed and the result is deterministic).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -128,9 +129,10 @@ __global__ __launch_bounds__(TX *RTY) void k_residual(GlideGeom g, Params p, dou
                                                       const double *__restrict__ sub, const double *__restrict__ bed,
                                                       const double *__restrict__ B, const double *__restrict__ beta,
                                                       const double *__restrict__ mask,
-                                                      const double *__restrict__ gamma)
+                                                      const double *__restrict__ gamma, int *__restrict__ zero)
 {
     __shared__ ResidLDS<TX, RTY> S;
+    if (zero && blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) *zero = 0;   // a flag the next kernel raises
     const int tx = cdiv(g.nx, TX), total = tx * cdiv(g.ny, RTY);   // cell tiles; edge faces by the last tiles
     tile_loop(total, [&](int t) {
         __syncthreads();   // the previous tile's readers of S are done
@@ -345,8 +347,6 @@ __global__ __launch_bounds__(256) void k_local_activation(int ny, int nx, const 
     activation_block(ny, nx, r, U, gamma, blk, blockIdx.x, blockIdx.y, gridDim.x, any_active);
 }
 
-__global__ void k_set_int(int *__restrict__ x, int v) { *x = v; }
-
 // ---------------------------------------------------------------------------
 // Transfer operators (glide/cuda/utility.cu)
 // ---------------------------------------------------------------------------
@@ -397,11 +397,14 @@ __global__ void k_restrict_max(const double *__restrict__ hf, double *__restrict
     hc[idx] = fmax(a, b);
 }
 // The three restrictions of a state vector [u | v | H] in one launch (the
-// arithmetic of k_restrict_u, k_restrict_v and k_restrict_cell per point).
+// arithmetic of k_restrict_u, k_restrict_v and k_restrict_cell per point);
+// optionally also a[0, n_fill) = fill_val (the launch covers both ranges).
 __global__ void k_restrict_state(const double *__restrict__ src, double *__restrict__ dst, double *__restrict__ dst2,
-                                 int nyc, int nxc, double bw)
+                                 int nyc, int nxc, double bw, double *__restrict__ fill_a, size_t n_fill,
+                                 double fill_val)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n_fill) fill_a[idx] = fill_val;
     const size_t nuc = n_u(nyc, nxc), nvc = n_v(nyc, nxc), ncc = n_c(nyc, nxc);
     const size_t nxf = 2 * (size_t)nxc, nuf = n_u(2 * nyc, 2 * nxc), nvf = n_v(2 * nyc, 2 * nxc);
     double val;
@@ -518,9 +521,11 @@ __global__ void k_gamma_pre(int ny, int nx, double *__restrict__ gamma, const do
 // post-correction k_gamma, gamma = w_H + chi (it reads no U).
 __global__ void k_fas_correct(int ny, int nx, double *__restrict__ U, const double *__restrict__ w,
                               const double *__restrict__ Uc, const double *__restrict__ wc,
-                              double *__restrict__ gamma, const double *__restrict__ chi)
+                              double *__restrict__ gamma, const double *__restrict__ chi,
+                              double *__restrict__ gamma_c, double thklim)
 {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gamma_c && k < n_c(ny / 2, nx / 2)) gamma_c[k] = thklim;   // the child's reset (fewer cells than nt)
     const size_t nu = n_u(ny, nx), nv = n_v(ny, nx), nc = n_c(ny, nx);
     if (k >= nu + nv + nc) return;
     const int nyc = ny / 2, nxc = nx / 2;
@@ -1119,20 +1124,21 @@ __global__ void k_grad_B(GlideGeom g, Params p, double *__restrict__ grad, const
 // ===========================================================================
 namespace {
 
+// zero (optional): an int the launch also sets to 0
 int launch_residual(const GlideGeom &g, const Params &p, double *r, const double *U, const double *f, const double *sub,
                     const double *bed, const double *B, const double *beta, const double *mask, const double *gamma,
-                    hipStream_t st)
+                    hipStream_t st, int *zero = nullptr)
 {
     const int ty = residual_ty(g.ny, g.nx);
     if (ty == 16)
         hipLaunchKernelGGL(k_residual<16>, residual_grid(g.ny, g.nx, 16), dim3(TX, 16), 0, st, g, p, r, U, f, sub, bed,
-                           B, beta, mask, gamma);
+                           B, beta, mask, gamma, zero);
     else if (ty == 8)
         hipLaunchKernelGGL(k_residual<8>, residual_grid(g.ny, g.nx, 8), dim3(TX, 8), 0, st, g, p, r, U, f, sub, bed, B,
-                           beta, mask, gamma);
+                           beta, mask, gamma, zero);
     else
         hipLaunchKernelGGL(k_residual<4>, residual_grid(g.ny, g.nx, 4), dim3(TX, 4), 0, st, g, p, r, U, f, sub, bed, B,
-                           beta, mask, gamma);
+                           beta, mask, gamma, zero);
     return check_launch("residual");
 }
 
@@ -1282,14 +1288,16 @@ int read_flag(GlideVcycleWork *w, int k, bool *any)
     return GLIDE_OK;
 }
 
-// F is the 2x refinement of C (check_levels)
+// F is the 2x refinement of C (check_levels); fill_a (optional) receives
+// fill_val in its first n_fill entries in the same launch.
 int restrict_state(const GlideLevel &F, const GlideLevel &C, const double *src, double *dst, double *dst2,
-                   hipStream_t st, double bw = 1.0)
+                   hipStream_t st, double bw = 1.0, double *fill_a = nullptr, size_t n_fill = 0, double fill_val = 0.0)
 {
     (void)F;
     const int nyc = C.g.ny, nxc = C.g.nx;
-    hipLaunchKernelGGL(k_restrict_state, dim3(blocks1d(n_t(nyc, nxc))), dim3(256), 0, st, src, dst, dst2, nyc, nxc,
-                       bw);
+    const size_t n = std::max(n_t(nyc, nxc), fill_a ? n_fill : 0);
+    hipLaunchKernelGGL(k_restrict_state, dim3(blocks1d(n)), dim3(256), 0, st, src, dst, dst2, nyc, nxc, bw, fill_a,
+                       fill_a ? n_fill : 0, fill_val);
     return check_launch("restrict state");
 }
 
@@ -1353,7 +1361,7 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
     if (k == n - 1) {
         hipLaunchKernelGGL(k_gamma, dim3(blocks1d(nc)), dim3(256), 0, st, ny, nx, A.gamma, wH, A.chi, nullptr);
         TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, nullptr, sc.coarse, sc.n_newton, A.scratch, st));
-        return fill(A.gamma, nc, thklim, st);
+        return k > 0 ? GLIDE_OK : fill(A.gamma, nc, thklim, st);   // a parent resets gamma in its FAS correction
     }
     const GlideLevel &C = L[k + 1];
     const int nyc = C.g.ny, nxc = C.g.nx;
@@ -1361,15 +1369,15 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
     hipLaunchKernelGGL(k_gamma_pre, dim3(blocks1d(nc)), dim3(256), 0, st, ny, nx, A.gamma, wH, A.chi, C.chi);
     TRY(check_launch("pre gamma"));
     TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, nullptr, sc.pre, sc.n_newton, A.scratch, st));
-    TRY(fill(A.gamma, nc, thklim, st));
-    TRY(restrict_state(A, C, A.U, C.U, C.w, st));
+    TRY(restrict_state(A, C, A.U, C.U, C.w, st, 1.0, A.gamma, nc, thklim));   // and gamma = thklim
     TRY(launch_residual(A.g, p, A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma, st));
     TRY(restrict_state(A, C, A.r, C.r, nullptr, st, sc.boundary_weight));
     // f_c = F_c(U_c) - r_c
     TRY(launch_residual(C.g, pc, C.f, C.U, nullptr, C.r, C.bed, C.B, C.beta, C.mask, C.gamma, st));
     TRY(fascd(L, n, k + 1, false, thklim, sc, work, st));
+    // also the post-correction gamma and the child's gamma = thklim
     hipLaunchKernelGGL(k_fas_correct, dim3(blocks1d(nt)), dim3(256), 0, st, ny, nx, A.U, A.w, C.U, C.w, A.gamma,
-                       A.chi);
+                       A.chi, C.gamma, thklim);
     TRY(check_launch("fas correct"));
     TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, nullptr, sc.post, sc.n_newton, A.scratch, st));
     // A pass whose activation marks no block leaves U unchanged (its apply
@@ -1390,10 +1398,11 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
     GlideVcycleWork *chk = sc.local_skip && sc.local_passes > 0 ? work : nullptr;
     int *flags = reinterpret_cast<int *>(A.scratch);
     for (int pass = 0; pass < sc.local_passes; pass++) {
-        TRY(launch_residual(A.g, p, A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma, st));
+        // the residual launch also clears this pass's flag slot
+        TRY(launch_residual(A.g, p, A.r, A.U, A.f, nullptr, A.bed, A.B, A.beta, A.mask, A.gamma, st,
+                            chk ? flags + (pass & 1) : nullptr));
         if (chk) {
             const int k = pass & 1;
-            hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, flags + k, 0);
             TRY(launch_activation(A.g, A.r, A.U, A.gamma, A.active, st, flags + k));
             TRY(post_flag(chk, k, flags + k, st));
             bool any = true;
@@ -1412,7 +1421,7 @@ int fascd(const GlideLevel *L, int n, int k, bool finest, double thklim, const G
         TRY(launch_sweeps(A.g, p, A.U, A.f, A.bed, A.B, A.beta, A.gamma, A.active, sc.local_sweeps, sc.n_newton,
                           A.scratch, st));
     }
-    return fill(A.gamma, nc, thklim, st);
+    return k > 0 ? GLIDE_OK : fill(A.gamma, nc, thklim, st);   // a parent resets gamma in its FAS correction
 }
 
 
